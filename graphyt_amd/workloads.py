@@ -1,0 +1,190 @@
+"""Synthetic MPM workloads (scenes) with the sizes of the benchmark configurations.
+
+A `Scene` is everything `gy_create` / `gy_set_node_bc` / `gy_upload` need: grid, material
+table, run parameters, per-particle initial fields and per-node boundary conditions.
+Random draws use numpy's default_rng with a recorded seed.
+
+  * `elastic_cube`      — 3D periodic box with an elastic cube in shear-free compression
+                          (0.5 m box, h = 0.02, 20^3 particles at psep = h/2, 500 steps).
+  * `dp_column`         — 3D Drucker-Prager granular column collapse
+                          (2x1x1 m, h = 0.01, 80x160x80 = 1,024,000 particles, 1000 steps),
+                          and scaled-down versions of it for CPU parity runs.
+  * `impact_block`      — elastic/perfect-plastic block impacting a fixed wall (ensemble member).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .abi import BC, GyGrid, GyMaterial, GyParams, ParticleArrays, material
+
+SEED = 20261203
+
+
+@dataclass
+class Scene:
+    name: str
+    L: tuple
+    h: float
+    psep: float
+    mats: list                      # list[GyMaterial]
+    gravity: tuple = (0.0, 0.0, 0.0)
+    damping: float = 0.0
+    mu: float = 0.0
+    cfl: float = 0.1
+    dt: float = 0.0                 # <= 0: the CFL rule of Solver::Solver
+    steps: int = 100
+    particles: ParticleArrays = None
+    node_bc_type: np.ndarray = None  # int32[numNodes]
+    node_bc_val: np.ndarray = None   # float64[numNodes, 4]
+    meta: dict = field(default_factory=dict)
+
+    # -- grid geometry (Nodes.cpp:12, 71-73) --
+    @property
+    def dims(self):
+        return tuple(int(1 + (self.L[k] / self.h)) for k in range(3))
+
+    @property
+    def num_nodes(self):
+        L, h = self.L, self.h
+        return int(round((1 + (L[0] / h)) * (1 + (L[1] / h)) * (1 + (L[2] / h))))
+
+    @property
+    def n(self):
+        return self.particles.n
+
+    def grid(self) -> GyGrid:
+        g = GyGrid()
+        for k in range(3):
+            g.L[k] = self.L[k]
+        g.cellsize = self.h
+        return g
+
+    def params(self) -> GyParams:
+        p = GyParams()
+        for k in range(3):
+            p.gravity[k] = self.gravity[k]
+        p.damping, p.mu, p.cfl, p.psep, p.dt, p.is3D = self.damping, self.mu, self.cfl, self.psep, self.dt, 1
+        return p
+
+    def resolved_dt(self) -> float:
+        """dt as Solver::Solver sets it (Solver.cpp:15-32), unless overridden by setDt."""
+        if self.dt > 0:
+            return self.dt
+        maxK, minrho = 1.0, 1e15
+        for m in self.mats:
+            maxK = max(maxK, m.bulk_mod)
+            minrho = min(minrho, m.density)
+        return self.cfl * (self.h / math.sqrt(maxK / minrho))
+
+    def node_positions(self) -> np.ndarray:
+        Nx, Ny, Nz = self.dims
+        k, j, i = np.meshgrid(np.arange(Nz), np.arange(Ny), np.arange(Nx), indexing="ij")
+        pos = np.stack([i.ravel() * self.h, j.ravel() * self.h, k.ravel() * self.h], axis=1)
+        return pos
+
+
+def lattice(lo, counts, psep, jitter=0.0, seed=SEED):
+    """(i+1/2)*psep lattice points from corner `lo`, x fastest, optional U(-j, j)*psep jitter."""
+    nx, ny, nz = counts
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([lo[0] + (i.ravel() + 0.5) * psep,
+                    lo[1] + (j.ravel() + 0.5) * psep,
+                    lo[2] + (k.ravel() + 0.5) * psep], axis=1)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape) * psep
+    return pos
+
+
+def _particles(pos, vel=None, material=None, obj=None):
+    n = pos.shape[0]
+    pa = ParticleArrays(n)
+    pa.arrays["pos"] = np.ascontiguousarray(pos, dtype=np.float64)
+    pa.arrays["vel"] = np.zeros((n, 3)) if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+    pa.arrays["material"] = np.zeros(n, np.int32) if material is None else np.asarray(material, np.int32)
+    pa.arrays["obj_id"] = np.zeros(n, np.int32) if obj is None else np.asarray(obj, np.int32)
+    return pa
+
+
+def elastic_cube(h=0.02, L=0.5, side=0.2, steps=500) -> Scene:
+    """Periodic box with an elastic cube, v_z = -0.5 (z - z_c)/0.1, gravity -9.81 z.
+
+    dt = 0.2 h / c_p with c_p = sqrt((K + 4G/3)/rho), set through setDt.
+    """
+    psep = h / 2
+    n = int(round(side / psep))
+    lo = (L - side) / 2
+    pos = lattice((lo, lo, lo), (n, n, n), psep)
+    zc = L / 2
+    vel = np.zeros_like(pos)
+    vel[:, 2] = -0.5 * (pos[:, 2] - zc) / 0.1
+    K, G, rho = 1e6, 5e5, 1000.0
+    m = material(model=1, bulk_mod=K, shear_mod=G, density=rho)
+    s = Scene("elastic_cube", (L, L, L), h, psep, [m], gravity=(0, 0, -9.81),
+              dt=0.2 * h / math.sqrt((K + 4 * G / 3) / rho), steps=steps)
+    s.particles = _particles(pos, vel)
+    s.node_bc_type = np.zeros(s.num_nodes, np.int32)
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    return s
+
+
+def dp_column(h=0.01, L=(2.0, 1.0, 1.0), lo=(0.03, 0.03, 0.30), size=(0.4, 0.8, 0.4),
+              steps=1000, jitter=0.05, cohesion=0.0, phi=30.0, psi=0.0, seed=SEED,
+              apex_guard=True) -> Scene:
+    """Drucker-Prager granular column collapse (model 4, rho 2000, K 5e7, G 2e7).
+
+    Floor nodes (y <= 2h) get grid_vx_vy [0, 0]; wall nodes (x <= 2h or x >= L-2h) get
+    grid_vx [0]; the floor wins where both apply. Gravity -9.81 along y; z is periodic
+    because z-direction grid BCs are no-ops in the reference (Grid.cpp:872-874, 1032-1052).
+
+    With c = 0 and psi = 0 the reference's apex return evaluates 0*inf (NaN) for every
+    yielding particle in tension (ConstitutiveModels.cpp:364-365); `apex_guard` (default on)
+    selects the guarded apex that returns sigma = 0 there, in the oracle and on the GPU alike.
+    """
+    psep = h / 2
+    counts = tuple(int(round(size[k] / psep)) for k in range(3))
+    pos = lattice(lo, counts, psep, jitter=jitter, seed=seed)
+    m = material(model=4, bulk_mod=5e7, shear_mod=2e7, density=2000.0, cohesion=cohesion,
+                 phi=phi, psi=psi, critical_strain=0.01, dp_apex_guard=apex_guard)
+    s = Scene("dp_column", tuple(L), h, psep, [m], gravity=(0, -9.81, 0), steps=steps)
+    s.particles = _particles(pos)
+    npos = s.node_positions()
+    wall = 2.0 * h
+    t = np.zeros(s.num_nodes, np.int32)
+    t[(npos[:, 0] <= wall) | (npos[:, 0] >= L[0] - wall)] = BC["grid_vx"]
+    t[npos[:, 1] <= wall] = BC["grid_vx_vy"]
+    s.node_bc_type = t
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    s.meta.update(seed=seed, counts=counts)
+    return s
+
+
+def dp_column_small(h=0.02, steps=50, **kw) -> Scene:
+    """A CPU-sized Drucker-Prager column (same material/BCs, 20x30x10 particles)."""
+    return dp_column(h=h, L=(0.6, 0.5, 0.4), lo=(0.06, 0.06, 0.1), size=(0.2, 0.3, 0.1),
+                     steps=steps, **kw)
+
+
+def impact_block(h=0.01, n_nodes=24, count=20000, K=1e11, G=4e10, sy=5e8, v=100.0,
+                 model=3, seed=SEED) -> Scene:
+    """Perfect-plastic steel block moving at -v along x toward a fixed grid_vx wall."""
+    L = (n_nodes - 1) * h
+    psep = h / 2
+    side = max(1, int(round(count ** (1.0 / 3.0))))
+    size = side * psep
+    lo = (3 * h, (L - size) / 2, (L - size) / 2)
+    pos = lattice(lo, (side, side, side), psep, jitter=0.0, seed=seed)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = -v
+    m = material(model=model, bulk_mod=K, shear_mod=G, density=7800.0, yield_stress=sy)
+    s = Scene("impact_block", (L, L, L), h, psep, [m], steps=500)
+    s.particles = _particles(pos, vel)
+    npos = s.node_positions()
+    t = np.zeros(s.num_nodes, np.int32)
+    t[npos[:, 0] <= 2 * h] = BC["grid_vx"]
+    s.node_bc_type = t
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    return s

@@ -1,0 +1,215 @@
+/*
+ * graphyt_amd.h — C ABI of the MI355X (gfx950) MPM time step.
+ *
+ * This is the drop-in boundary for Graphyt's `Solver::iterate_GPU(double t, int step,
+ * Parameters*)` slot (Solver.hpp:21, Solver.i:16, stub at Solver.cpp:293-302): plain C,
+ * POD arguments, int status codes, no torch or HIP types in any signature. Host code
+ * (the C++ `graphyt` classes, a ctypes/cgo/JNI binding) owns FP64 host arrays; the
+ * context owns FP32 device state and replays the step on its own HIP stream.
+ *
+ * Field layouts follow the reference: vectors interleaved per particle
+ * (`pos[3p+k]`), symmetric tensors as [xx, yy, zz, xy, yz, xz]
+ * (MaterialPoints.cpp:312-318, DamageModels.cpp:73-81), nodes numbered
+ * n = i + j*Nx + k*Nx*Ny (Nodes.cpp:77-89).
+ */
+#ifndef GRAPHYT_AMD_H
+#define GRAPHYT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------- */
+#define GY_OK              0
+#define GY_ERR_INVALID     1   /* bad argument or inconsistent sizes                   */
+#define GY_ERR_UNSUPPORTED 2   /* a feature outside the implemented path (2D, model 5) */
+#define GY_ERR_HIP         3   /* a HIP runtime call failed                            */
+#define GY_ERR_RANGE       4   /* a particle's GIMP stencil left the grid              */
+#define GY_ERR_STATE       5   /* call order violated (e.g. iterate before upload)     */
+
+/* ---- boundary-condition codes: BoundaryConditions.hpp:6-77 (values are ABI) ------- */
+enum gy_bc_type {
+  GY_BC_NONE = 0,
+  GY_BC_GRID_VX = 1, GY_BC_GRID_VY = 2, GY_BC_GRID_VZ = 3, GY_BC_GRID_TEMP = 4,
+  GY_BC_GRID_TFLUX_X = 5, GY_BC_GRID_TFLUX_Y = 6, GY_BC_GRID_TFLUX_Z = 7,
+  GY_BC_GRID_FX = 8, GY_BC_GRID_FY = 9, GY_BC_GRID_FZ = 10,
+  GY_BC_GRID_VX_VY = 11, GY_BC_GRID_VX_VZ = 12, GY_BC_GRID_VY_VZ = 13,
+  GY_BC_GRID_FX_FY = 14, GY_BC_GRID_FY_FZ = 15, GY_BC_GRID_FX_FZ = 16, GY_BC_GRID_FX_FY_FZ = 17,
+  GY_BC_GRID_VX_VY_VZ = 18,
+  /* 19..32: grid_*_temp variants (no-ops in the reference) */
+  GY_BC_MP_VX = 33, GY_BC_MP_VY = 34, GY_BC_MP_VZ = 35, GY_BC_MP_TEMP = 36,
+  GY_BC_MP_FX = 37, GY_BC_MP_FY = 38, GY_BC_MP_FZ = 39,
+  GY_BC_MP_VX_VY = 40, GY_BC_MP_VX_VZ = 41, GY_BC_MP_VY_VZ = 42,
+  GY_BC_MP_FX_FY = 43, GY_BC_MP_FY_FZ = 44, GY_BC_MP_FX_FZ = 45, GY_BC_MP_FX_FY_FZ = 46,
+  GY_BC_MP_VX_VY_VZ = 47,
+  /* 48..61: mp_*_temp variants (no-ops in the reference) */
+  GY_BC_MP_STRESS = 62, GY_BC_MP_STRESS_XX_YY = 63, GY_BC_MPSTRAIN = 64,
+  GY_BC_MP_VXSIN = 65, GY_BC_MP_VYSIN = 66, GY_BC_MP_SYYSIN = 67
+};
+
+/* ---- plain-old-data descriptions ------------------------------------------------- */
+
+/* Background grid: Nodes(L, cellsize) (Nodes.cpp:8-141). Node counts follow
+ * Nx = int(1 + L/h) (Nodes.cpp:71-73); L/h must be an integer (Nodes.cpp:12 vs 71). */
+typedef struct gy_grid {
+  double L[3];
+  double cellsize;
+} gy_grid;
+
+/* One MaterialModels entry (MaterialModels.hpp:5-48, defaults MaterialModels.cpp:7-25). */
+typedef struct gy_material {
+  int32_t model;          /* 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager */
+  int32_t is_rigid;
+  int32_t is_damage;
+  int32_t damage_model;   /* 1 = Grady-Kipp */
+  double  bulk_mod;
+  double  shear_mod;
+  double  density;
+  double  viscosity;
+  double  yield_stress;
+  double  cohesion;
+  double  critical_strain;
+  double  phi_deg;        /* friction angle, degrees */
+  double  psi_deg;        /* dilatancy angle, degrees */
+  double  crack_speed;    /* Cg */
+  double  damage_m;
+  double  damage_k;
+  double  thermal_conductivity;   /* carried for completeness; thermal path not simulated */
+  double  specific_heat;
+  double  turb_stress;
+  /* Drucker-Prager apex guard (not in the reference): when nonzero, the products
+   * cval*(xi/eta_bar) and dcval*(xi/eta_bar) of the apex return
+   * (ConstitutiveModels.cpp:364-365) are taken as 0 when cval (dcval) is 0, so a
+   * cohesionless, non-dilatant material (c = 0, psi = 0) returns to sigma = 0 instead of
+   * the reference's 0*inf = NaN. 0 = reference behaviour. */
+  int32_t dp_apex_guard;
+  int32_t reserved;
+} gy_material;
+
+/* Parameters (Parameters.hpp:16-32) plus the particle spacing MaterialPoints::psep and
+ * the step size Solver::dt (Solver.cpp:32 or Solver::setDt). */
+typedef struct gy_params {
+  double  gravity[3];
+  double  damping;        /* velocityDampingCoef */
+  double  mu;             /* contact friction coefficient */
+  double  cfl;
+  double  psep;
+  double  dt;
+  int32_t is3D;           /* must be 1 */
+  int32_t reserved;
+} gy_params;
+
+/* Host view of particle fields; any pointer may be NULL (field skipped on upload or
+ * download; defaults on upload are the values MaterialPoints::add pushes, i.e. zero,
+ * material 0, object 0, no hold, no BC). Arrays are indexed by the caller's particle
+ * order, which the context preserves across steps. */
+typedef struct gy_particles {
+  int32_t  n;
+  int32_t  reserved;
+  double  *pos;                /* 3n */
+  double  *pos0;               /* 3n: reference position for displacement       */
+  double  *vel;                /* 3n */
+  double  *stress;             /* 6n */
+  double  *strain;             /* 6n */
+  double  *plastic_strain;     /* 6n */
+  double  *acc_plastic_strain; /* n  */
+  double  *density;            /* n  */
+  double  *mass;               /* n  */
+  double  *damage;             /* n  */
+  double  *smax;               /* n  */
+  int32_t *material;           /* n  */
+  int32_t *obj_id;             /* n  */
+  uint8_t *hold;               /* n  */
+  int32_t *bc_type;            /* n  */
+  double  *bc_val;             /* 4n */
+  /* download-only derived fields */
+  double  *volume;             /* n  : mass / density                       */
+  double  *displacement;       /* 3n : pos - pos0                           */
+  int32_t *cell_id;            /* n  : reference cellID (TimeIntegration.cpp:110-115) */
+} gy_particles;
+
+/* FP64 audit of the device state (sums over all particles of every instance). */
+typedef struct gy_audit_out {
+  double  mass;
+  double  momentum[3];
+  double  kinetic_energy;
+  double  strain_energy_proxy;  /* 0.5 * V * sigma:sigma / (2G): bounded check value */
+  int64_t nonfinite;            /* particles with a NaN/Inf in x, v or sigma */
+  int64_t out_of_range;         /* particles whose stencil left the grid (clamped) */
+} gy_audit_out;
+
+typedef struct gy_ctx gy_ctx;
+
+/* ---- lifecycle -------------------------------------------------------------------- */
+
+/* Solver::Solver + Solver::initialize (Solver.cpp:7-69) for one instance. `params->dt`
+ * <= 0 selects the reference rule dt = CFL*h/sqrt(maxK/minRho) (Solver.cpp:15-32). */
+int  gy_create(gy_ctx **out, const gy_grid *grid, const gy_material *mats, int32_t nmat,
+               const gy_params *params);
+
+/* Nodes::setBC for every node at once (Nodes.cpp:155-163): bc_type[n], bc_val[4n..4n+3]. */
+int  gy_set_node_bc(gy_ctx *ctx, int32_t n_nodes, const int32_t *bc_type, const double *bc_val);
+
+/* Upload particle state (MaterialPoints::add + setters, MaterialPoints.cpp:95-339).
+ * If `mass` or `density` is NULL they are set as Solver::initialize does
+ * (rho = rho_mat, V = psep^3, m = rho*V, rigid m = 1; Solver.cpp:42-56). */
+int  gy_upload(gy_ctx *ctx, const gy_particles *host);
+
+/* Solver::setDt (Solver.cpp:70-73). */
+int  gy_set_dt(gy_ctx *ctx, double dt);
+double gy_get_dt(const gy_ctx *ctx);
+
+/* One Solver::iterate_CPU-equivalent step (Solver.cpp:86-292), enqueued on the context
+ * stream; returns without waiting. */
+int  gy_iterate(gy_ctx *ctx, double t, int32_t step);
+
+/* `nsteps` steps with t = t0 + i*dt, step = step0 + i (Solver::iterate_CPU_steps,
+ * Solver.cpp:75-84, with the time passed explicitly). */
+int  gy_iterate_n(gy_ctx *ctx, double t0, int32_t step0, int32_t nsteps);
+
+/* Wait for the context stream; reports a deferred device-side error (GY_ERR_RANGE). */
+int  gy_synchronize(gy_ctx *ctx);
+
+/* Download particle fields (FP32 device state widened to FP64) in the caller's order. */
+int  gy_download(gy_ctx *ctx, gy_particles *host);
+
+/* FP64 mass / momentum / energy audit reduction on the device. */
+int  gy_audit(gy_ctx *ctx, gy_audit_out *out);
+
+/* Binning of the current particle positions (the counting sort the next step uses):
+ * keys[s] for device slot s, counts/offsets per key (nkeys entries), and the stable
+ * order (order[i] = slot). Any pointer may be NULL; *nkeys receives the key count. */
+int  gy_debug_binning(gy_ctx *ctx, int32_t *keys, int32_t *counts, int32_t *offsets,
+                      int32_t *order, int64_t *nkeys, int64_t *nslots);
+
+/* Standalone stable counting sort of host keys in [0, nkeys) with the same kernels the
+ * step uses: counts, exclusive offsets and order (order[i] = index of the i-th key). */
+int  gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys,
+                      int32_t *counts, int32_t *offsets, int32_t *order);
+
+/* Options: see GY_OPT_*. */
+#define GY_OPT_USE_GRAPH   1   /* replay gy_iterate_n through a captured hipGraph (default 1) */
+#define GY_OPT_PROFILE     2   /* record HIP events around every kernel (default 0)          */
+#define GY_OPT_TILE        3   /* tile shape id for the particle kernels (see gy_tile_shape) */
+int  gy_set_option(gy_ctx *ctx, int32_t option, int64_t value);
+
+/* Per-kernel accumulated device time since the last reset (GY_OPT_PROFILE on).
+ * `name` is one of the kernel names listed by gy_kernel_name(i). */
+int  gy_kernel_count(void);
+const char *gy_kernel_name(int32_t i);
+int  gy_kernel_stats(gy_ctx *ctx, int32_t i, int64_t *launches, double *total_ms);
+int  gy_reset_kernel_stats(gy_ctx *ctx);
+
+/* The context's HIP stream (a hipStream_t), for callers that time on it. */
+void *gy_stream(gy_ctx *ctx);
+
+const char *gy_last_error(const gy_ctx *ctx);
+const char *gy_version(void);
+void gy_destroy(gy_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPHYT_AMD_H */
