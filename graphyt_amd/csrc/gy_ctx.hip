@@ -1,0 +1,745 @@
+// gy_ctx.hip — host side of the C ABI (include/graphyt_amd.h): context, device memory,
+// upload/download between the caller's FP64 arrays and the FP32 device state, the
+// step sequence and its hipGraph replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/graphyt_amd.h"
+#include "gy_internal.hpp"
+
+using namespace gy;
+
+namespace {
+
+const char *kKernelNames[K_COUNT_] = {"keys",  "scan_partial", "scan_final", "place",   "p2g",
+                                      "grid",  "g2p2g",        "advance",    "audit",   "order"};
+
+struct Timed {
+  int id;
+  hipEvent_t a, b;
+};
+
+// Per-instance host information kept for upload/download conversions.
+struct InstHost {
+  gy_grid grid;
+  gy_params params;
+  std::vector<gy_material> mats;
+  int n[3];
+  double dt;
+  int nparticles = 0;
+  std::vector<double> pos0;  // caller's reference positions (displacement)
+  std::vector<uint8_t> carry_plastic, carry_damage;
+};
+
+}  // namespace
+
+struct gy_ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  std::vector<InstHost> ih;
+  std::vector<InstDesc> inst;
+  std::vector<MatDev> mats;
+  DevPtrs d{};
+  long long nnodes = 0;
+  bool uploaded = false;
+  bool acc_valid = false;
+  int host_step = 0;
+  // options
+  bool use_graph = true;
+  bool profile = false;
+  hipGraphExec_t graph = nullptr;
+  bool graph_valid = false;
+  std::vector<Timed> pending;
+  int64_t launches[K_COUNT_] = {0};
+  double total_ms[K_COUNT_] = {0};
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+int fail(gy_ctx *c, int code, const char *fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                 \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(ctx, GY_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+template <class T>
+int dalloc(gy_ctx *c, T **p, size_t count) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (count == 0) count = 1;
+  HIPCHECK(c, hipMalloc((void **)p, count * sizeof(T)));
+  HIPCHECK(c, hipMemsetAsync(*p, 0, count * sizeof(T), c->stream));
+  return GY_OK;
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+hipEvent_t get_event(gy_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Launch with optional per-kernel event timing.
+template <class F>
+void timed(gy_ctx *c, int id, F &&launch) {
+  if (!c->profile) {
+    launch();
+    return;
+  }
+  Timed t{id, get_event(c), get_event(c)};
+  (void)hipEventRecord(t.a, c->stream);
+  launch();
+  (void)hipEventRecord(t.b, c->stream);
+  c->pending.push_back(t);
+}
+
+void collect_timing(gy_ctx *c) {
+  for (auto &t : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      c->launches[t.id] += 1;
+      c->total_ms[t.id] += ms;
+    }
+    c->event_pool.push_back(t.a);
+    c->event_pool.push_back(t.b);
+  }
+  c->pending.clear();
+}
+
+int fixed_bits(const int n[3]) {
+  int nmax = std::max(n[0], std::max(n[1], n[2])) + 8;
+  int b = 0;
+  while ((1 << b) < nmax) b++;
+  return std::min(23, 31 - b);
+}
+
+MatDev make_mat(const gy_material &m) {
+  MatDev M{};
+  M.model = m.model;
+  M.rigid = m.is_rigid != 0;
+  M.damage = m.is_damage != 0;
+  M.gk = (m.is_damage != 0) && m.damage_model == 1;
+  M.apex_guard = m.dp_apex_guard != 0;
+  M.carry_plastic = (m.model == 3 || m.model == 4);
+  M.carry_damage = M.gk;
+  M.K = (float)m.bulk_mod;
+  M.G = (float)m.shear_mod;
+  M.visc = (float)m.viscosity;
+  M.sy = (float)m.yield_stress;
+  // Drucker-Prager constants, ConstitutiveModels.cpp:246-295 (pi = 3.14159 as there)
+  const double phi = m.phi_deg * 3.14159 / 180.0, psi = m.psi_deg * 3.14159 / 180.0;
+  const double eta = 6.0 * sin(phi) / (sqrt(3.0) * (3.0 - sin(phi)));
+  const double xi = 6.0 * cos(phi) / (sqrt(3.0) * (3.0 - sin(phi)));
+  const double etab = (phi == psi) ? eta : 6.0 * sin(psi) / (sqrt(3.0) * (3.0 - sin(psi)));
+  M.c = (float)m.cohesion;
+  M.ecr = (float)m.critical_strain;
+  M.eta = (float)eta;
+  M.xi = (float)xi;
+  M.eta_bar = (float)etab;
+  M.c_over_ecr = (float)(m.cohesion / m.critical_strain);
+  M.c_floor = (float)(m.cohesion / 10.0);
+  M.xi_over_eta = (float)(xi / eta);
+  M.xi_over_etab = (float)(xi / etab);
+  // Grady-Kipp constants, DamageModels.cpp:96-106
+  const double Cg = m.crack_speed, dm = m.damage_m, dk = m.damage_k;
+  M.gk_K43 = m.bulk_mod + 4.0 * m.shear_mod / 3.0;
+  const double alpha = (8.0 * 3.14159 * Cg * Cg * Cg * dk) / (((dm + 1.0) * (dm + 2.0) * (dm + 3.0)));
+  M.gk_alpha13 = pow(alpha, 1.0 / 3.0);
+  M.gk_m = dm;
+  M.gk_k = dk;
+  M.gk_m3 = (dm + 3.0) / 3.0;
+  return M;
+}
+
+// Validate one instance and fill its descriptor (grid part).
+int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, const gy_params *p,
+             InstHost &H, InstDesc &I) {
+  if (!g || !mats || !p || nmat <= 0) return fail(c, GY_ERR_INVALID, "null grid/materials/params");
+  if (!p->is3D) return fail(c, GY_ERR_UNSUPPORTED, "only the 3D path is implemented");
+  if (nmat > 16) return fail(c, GY_ERR_UNSUPPORTED, "at most 16 materials per instance");
+  const double h = g->cellsize;
+  if (!(h > 0)) return fail(c, GY_ERR_INVALID, "cellsize must be positive");
+  for (int a = 0; a < 3; a++) {
+    const double r = g->L[a] / h;
+    if (std::fabs(r - std::round(r)) > 1e-9 * std::max(1.0, r))
+      return fail(c, GY_ERR_INVALID, "L[%d]/cellsize = %.12g is not an integer (Nodes.cpp:12 vs 71)", a, r);
+    I.n[a] = (int)(1 + (g->L[a] / h));
+    if (I.n[a] < 6) return fail(c, GY_ERR_INVALID, "need at least 6 nodes per axis for the periodic pairs");
+  }
+  if (!(p->psep > 0) || p->psep > h) return fail(c, GY_ERR_UNSUPPORTED, "psep must be in (0, cellsize]");
+  for (int i = 0; i < nmat; i++) {
+    const int md = mats[i].model;
+    if (md < 1 || md > 4) return fail(c, GY_ERR_UNSUPPORTED, "material %d: model %d not implemented", i, md);
+  }
+  H.grid = *g;
+  H.params = *p;
+  H.mats.assign(mats, mats + nmat);
+  for (int a = 0; a < 3; a++) H.n[a] = I.n[a];
+  // dt: Solver::Solver (Solver.cpp:15-32) unless given
+  double maxK = 1, minrho = 1e15;
+  for (int i = 0; i < nmat; i++) {
+    if (mats[i].bulk_mod > maxK) maxK = mats[i].bulk_mod;
+    if (mats[i].density < minrho) minrho = mats[i].density;
+  }
+  H.dt = p->dt > 0 ? p->dt : p->cfl * (h / sqrt(maxK / minrho));
+  const int F = fixed_bits(I.n);
+  I.fbits = F;
+  I.oneq = 1 << F;
+  I.lp = (float)(0.5 * p->psep / h);
+  I.inv_lp = (float)(h / (0.5 * p->psep));
+  I.lpq = (int)llround(0.5 * p->psep / h * (double)(1 << F));
+  for (int a = 0; a < 3; a++) {
+    I.cper[a] = I.n[a] - 3;
+    I.nt[a] = (I.n[a] - 2 + TILE - 1) / TILE;
+    I.nb[a] = (I.cper[a] + BRICK - 1) / BRICK;
+    I.wrapq[a] = (int)llround((g->L[a] - 2 * h) / h * (double)(1 << F));
+    I.gravity[a] = (float)p->gravity[a];
+  }
+  I.h = (float)h;
+  I.inv_h = (float)(1.0 / h);
+  I.dt = (float)H.dt;
+  I.qscale = (float)(H.dt * (double)(1 << F) / h);
+  I.damping = (float)p->damping;
+  I.inv_2F = (float)(1.0 / (double)(1 << F));
+  I.nmat = nmat;
+  return GY_OK;
+}
+
+int alloc_grid(gy_ctx *c) {
+  // global numbering across instances
+  long long node_base = 0;
+  int tile_base = 0, brick_base = 0, mat_base = 0;
+  c->mats.clear();
+  std::vector<int> tile_inst, brick_inst;
+  for (size_t i = 0; i < c->inst.size(); i++) {
+    InstDesc &I = c->inst[i];
+    I.node_base = node_base;
+    I.tile_base = tile_base;
+    I.brick_base = brick_base;
+    I.mat_base = mat_base;
+    const int nt = I.nt[0] * I.nt[1] * I.nt[2], nb = I.nb[0] * I.nb[1] * I.nb[2];
+    tile_inst.insert(tile_inst.end(), nt, (int)i);
+    brick_inst.insert(brick_inst.end(), nb, (int)i);
+    node_base += (long long)I.n[0] * I.n[1] * I.n[2];
+    tile_base += nt;
+    brick_base += nb;
+    for (auto &m : c->ih[i].mats) c->mats.push_back(make_mat(m));
+    mat_base += (int)c->ih[i].mats.size();
+  }
+  c->nnodes = node_base;
+  DevPtrs &d = c->d;
+  d.ninst = (int)c->inst.size();
+  d.ntiles = tile_base;
+  d.nbricks = brick_base;
+  d.nkeys = tile_base * TILE3;
+  d.nblocks_scan = (d.nkeys + scan_items_per_block() - 1) / scan_items_per_block();
+  int rc;
+  if ((rc = dalloc(c, &d.acc, (size_t)node_base * NODE_W))) return rc;
+  if ((rc = dalloc(c, &d.vel, (size_t)node_base * NODE_W))) return rc;
+  if ((rc = dalloc(c, &d.bc_type, (size_t)node_base))) return rc;
+  if ((rc = dalloc(c, &d.bc_val, (size_t)node_base * 2))) return rc;
+  if ((rc = dalloc(c, &d.brick_flag, (size_t)brick_base))) return rc;
+  if ((rc = dalloc(c, &d.tile_inst, (size_t)tile_base))) return rc;
+  if ((rc = dalloc(c, &d.brick_inst, (size_t)brick_base))) return rc;
+  if ((rc = dalloc(c, &d.count, (size_t)d.nkeys))) return rc;
+  if ((rc = dalloc(c, &d.offsets, (size_t)d.nkeys + 1))) return rc;
+  if ((rc = dalloc(c, &d.blocksum, (size_t)d.nblocks_scan))) return rc;
+  if ((rc = dalloc(c, &d.inst, c->inst.size()))) return rc;
+  if ((rc = dalloc(c, &d.mats, c->mats.size()))) return rc;
+  if ((rc = dalloc(c, &d.ctrl, 1))) return rc;
+  if ((rc = dalloc(c, &d.t_last, c->inst.size()))) return rc;
+  if ((rc = dalloc(c, &d.t_cur, c->inst.size()))) return rc;
+  HIPCHECK(c, hipMemcpyAsync(d.tile_inst, tile_inst.data(), tile_inst.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(d.brick_inst, brick_inst.data(), brick_inst.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(d.mats, c->mats.data(), c->mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return GY_OK;
+}
+
+int push_inst(gy_ctx *c) {
+  HIPCHECK(c, hipMemcpyAsync(c->d.inst, c->inst.data(), c->inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d.mats, c->mats.data(), c->mats.size() * sizeof(MatDev), hipMemcpyHostToDevice, c->stream));
+  return GY_OK;
+}
+
+int check_status(gy_ctx *c) {
+  Ctrl h{};
+  HIPCHECK(c, hipMemcpy(&h, c->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  if (h.status & 2) return fail(c, GY_ERR_STATE, "internal binning inconsistency (status %d)", h.status);
+  if (h.status & 1)
+    return fail(c, GY_ERR_RANGE, "%d particle(s) had a GIMP stencil outside the grid (clamped)", h.oor);
+  return GY_OK;
+}
+
+// One step after the binning of the current buffer is in place.
+void enqueue_step(gy_ctx *c) {
+  void *s = c->stream;
+  timed(c, K_GRID, [&] { launch_grid(c->d, s); });
+  timed(c, K_G2P2G, [&] { launch_g2p2g(c->d, s); });
+  timed(c, K_ADVANCE, [&] { launch_advance(c->d, s); });
+  timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+  timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+  timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+}
+
+// First step after an upload: bin the uploaded state and scatter it to the grid.
+void enqueue_init(gy_ctx *c) {
+  void *s = c->stream;
+  timed(c, K_KEYS, [&] { launch_keys(c->d, s); });
+  timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+  timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+  timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+  timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
+}
+
+void drop_graph(gy_ctx *c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_valid = false;
+}
+
+int build_graph(gy_ctx *c) {
+  drop_graph(c);
+  hipGraph_t g;
+  HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const bool prof = c->profile;
+  c->profile = false;
+  enqueue_step(c);
+  c->profile = prof;
+  HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, GY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  c->graph_valid = true;
+  return GY_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char *gy_version(void) { return "graphyt_amd 0.1 (gfx950)"; }
+
+const char *gy_last_error(const gy_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int gy_create(gy_ctx **out, const gy_grid *grid, const gy_material *mats, int32_t nmat, const gy_params *params) {
+  if (!out) return GY_ERR_INVALID;
+  *out = nullptr;
+  gy_ctx *c = new gy_ctx();
+  int rc;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GY_ERR_HIP;
+  }
+  c->ih.resize(1);
+  c->inst.resize(1);
+  memset(&c->inst[0], 0, sizeof(InstDesc));
+  if ((rc = describe(c, grid, mats, nmat, params, c->ih[0], c->inst[0])) || (rc = alloc_grid(c))) {
+    *out = c;  // keep the context so gy_last_error can report
+    return rc;
+  }
+  *out = c;
+  return GY_OK;
+}
+
+int gy_set_node_bc(gy_ctx *c, int32_t n_nodes, const int32_t *type, const double *val) {
+  if (!c || !type) return GY_ERR_INVALID;
+  if (n_nodes != c->nnodes) return fail(c, GY_ERR_INVALID, "expected %lld node BCs, got %d", c->nnodes, n_nodes);
+  std::vector<uint8_t> t(n_nodes);
+  std::vector<float> v(2 * (size_t)n_nodes, 0.f);
+  for (int i = 0; i < n_nodes; i++) {
+    if (type[i] < 0 || type[i] > GY_BC_MP_SYYSIN) return fail(c, GY_ERR_INVALID, "node %d: bad BC type %d", i, type[i]);
+    t[i] = (uint8_t)type[i];
+    if (val) {
+      v[2 * i] = (float)val[4 * i];
+      v[2 * i + 1] = (float)val[4 * i + 1];
+    }
+  }
+  HIPCHECK(c, hipMemcpy(c->d.bc_type, t.data(), t.size(), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->d.bc_val, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
+  return GY_OK;
+}
+
+int gy_upload(gy_ctx *c, const gy_particles *in) {
+  if (!c || !in || !in->pos || in->n <= 0) return fail(c, GY_ERR_INVALID, "upload needs n > 0 and positions");
+  InstHost &H = c->ih[0];
+  InstDesc &I = c->inst[0];
+  const int n = in->n;
+  const int nmat = (int)H.mats.size();
+  const double h = H.grid.cellsize, scale = (double)(1 << I.fbits) / h;
+  // multi-body contact (two velocity fields per node) is not on the single-body path
+  int obj0 = in->obj_id ? in->obj_id[0] : 0;
+  for (int i = 0; i < n; i++) {
+    const int m = in->material ? in->material[i] : 0;
+    if (m < 0 || m >= nmat) return fail(c, GY_ERR_INVALID, "particle %d: material %d out of range", i, m);
+    if (H.mats[m].is_rigid) return fail(c, GY_ERR_UNSUPPORTED, "rigid materials need two-body contact (not on this path)");
+    if (in->obj_id && in->obj_id[i] != obj0) return fail(c, GY_ERR_UNSUPPORTED, "more than one object ID needs two-body contact (not on this path)");
+    if (in->bc_type && (in->bc_type[i] < 0 || in->bc_type[i] > 127)) return fail(c, GY_ERR_INVALID, "particle %d: bad BC type", i);
+  }
+  const long long cap = n;
+  DevPtrs &d = c->d;
+  int rc;
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dalloc(c, &d.pbuf[b], (size_t)NF * cap))) return rc;
+    if ((rc = dalloc(c, &d.key[b], (size_t)cap))) return rc;
+  }
+  if ((rc = dalloc(c, &d.rank, (size_t)cap))) return rc;
+  if ((rc = dalloc(c, &d.tmp, (size_t)cap))) return rc;
+  if ((rc = dalloc(c, &d.pbc_val, 4 * (size_t)cap))) return rc;
+  if ((rc = dalloc(c, &d.audit, 8 * ((size_t)(cap + 255) / 256 + 1)))) return rc;
+  d.cap = cap;
+  d.np = n;
+  // host staging in the device layout
+  std::vector<uint32_t> buf((size_t)NF * cap, 0u);
+  auto F = [&](int f, int i) -> uint32_t & { return buf[(size_t)f * cap + i]; };
+  auto put = [&](int f, int i, double x) { float y = (float)x; std::memcpy(&F(f, i), &y, 4); };
+  H.carry_plastic.assign(nmat, 0);
+  H.carry_damage.assign(nmat, 0);
+  H.pos0.resize(3 * (size_t)n);
+  std::vector<float> pbc(4 * (size_t)n, 0.f);
+  const double psep3 = H.params.psep * H.params.psep * H.params.psep;
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) {
+      const double x = in->pos[3 * i + a];
+      const double q = std::nearbyint(x * scale);
+      if (!(std::fabs(q) < 2.0e9)) return fail(c, GY_ERR_RANGE, "particle %d: position out of the fixed-point range", i);
+      F(F_QX + a, i) = (uint32_t)(int32_t)q;
+      H.pos0[3 * i + a] = in->pos0 ? in->pos0[3 * i + a] : x;
+      if (in->vel) put(F_VX + a, i, in->vel[3 * i + a]);
+    }
+    const int m = in->material ? in->material[i] : 0;
+    for (int k = 0; k < 6; k++) {
+      if (in->stress) put(F_S0 + k, i, in->stress[6 * i + k]);
+      if (in->strain) put(F_E0 + k, i, in->strain[6 * i + k]);
+      if (in->plastic_strain) {
+        put(F_P0 + k, i, in->plastic_strain[6 * i + k]);
+        if (in->plastic_strain[6 * i + k] != 0.0) H.carry_plastic[m] = 1;
+      }
+    }
+    if (in->acc_plastic_strain) {
+      put(F_EPB, i, in->acc_plastic_strain[i]);
+      if (in->acc_plastic_strain[i] != 0.0) H.carry_plastic[m] = 1;
+    }
+    // Solver::initialize (Solver.cpp:42-56) when density/mass are not given
+    const double rho = in->density ? in->density[i] : H.mats[m].density;
+    const double mass = in->mass ? in->mass[i] : (H.mats[m].is_rigid ? 1.0 : rho * psep3);
+    put(F_RHO, i, rho);
+    put(F_MASS, i, mass);
+    if (in->damage) { put(F_DMG, i, in->damage[i]); if (in->damage[i] != 0.0) H.carry_damage[m] = 1; }
+    if (in->smax) { put(F_SMAX, i, in->smax[i]); if (in->smax[i] != 0.0) H.carry_damage[m] = 1; }
+    const int bct = in->bc_type ? in->bc_type[i] : 0;
+    F(F_TAG, i) = make_tag(m, in->hold ? (in->hold[i] ? 1 : 0) : 0, bct, in->obj_id ? in->obj_id[i] : 0);
+    F(F_PID, i) = (uint32_t)i;
+    if (in->bc_val)
+      for (int k = 0; k < 4; k++) pbc[4 * (size_t)i + k] = (float)in->bc_val[4 * i + k];
+  }
+  for (int m = 0; m < nmat; m++) {
+    MatDev &M = c->mats[I.mat_base + m];
+    const MatDev base = make_mat(H.mats[m]);
+    M.carry_plastic = base.carry_plastic || H.carry_plastic[m];
+    M.carry_damage = base.carry_damage || H.carry_damage[m];
+    H.carry_plastic[m] = (uint8_t)M.carry_plastic;
+    H.carry_damage[m] = (uint8_t)M.carry_damage;
+  }
+  H.nparticles = n;
+  I.pstart = 0;
+  I.pcount = n;
+  HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(d.pbc_val, pbc.data(), pbc.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if ((rc = push_inst(c))) return rc;
+  launch_set_ctrl(d, 0, 0.0, 1, c->stream);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->acc_valid = false;
+  drop_graph(c);
+  return GY_OK;
+}
+
+int gy_set_dt(gy_ctx *c, double dt) {
+  if (!c || !(dt > 0)) return GY_ERR_INVALID;
+  for (size_t i = 0; i < c->inst.size(); i++) {
+    c->ih[i].dt = dt;
+    InstDesc &I = c->inst[i];
+    I.dt = (float)dt;
+    I.qscale = (float)(dt * (double)(1 << I.fbits) / c->ih[i].grid.cellsize);
+  }
+  if (c->d.inst) {
+    int rc = push_inst(c);
+    if (rc) return rc;
+  }
+  return GY_OK;
+}
+
+double gy_get_dt(const gy_ctx *c) { return c && !c->ih.empty() ? c->ih[0].dt : 0.0; }
+
+static int iterate_impl(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
+  if (!c) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  if (nsteps <= 0) return GY_OK;
+  launch_set_ctrl(c->d, step0, t0, 0, c->stream);
+  if (!c->acc_valid) {
+    enqueue_init(c);
+    c->acc_valid = true;
+  }
+  if (c->use_graph && !c->profile && nsteps > 1) {
+    if (!c->graph_valid) {
+      int rc = build_graph(c);
+      if (rc) return rc;
+    }
+    for (int i = 0; i < nsteps; i++) HIPCHECK(c, hipGraphLaunch(c->graph, c->stream));
+  } else {
+    for (int i = 0; i < nsteps; i++) enqueue_step(c);
+  }
+  HIPCHECK(c, hipGetLastError());
+  c->host_step = step0 + nsteps;
+  return GY_OK;
+}
+
+int gy_iterate(gy_ctx *c, double t, int32_t step) { return iterate_impl(c, t, step, 1); }
+
+int gy_iterate_n(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
+  return iterate_impl(c, t0, step0, nsteps);
+}
+
+int gy_synchronize(gy_ctx *c) {
+  if (!c) return GY_ERR_INVALID;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (!c->uploaded) return GY_OK;
+  return check_status(c);
+}
+
+int gy_download(gy_ctx *c, gy_particles *o) {
+  if (!c || !o) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  int rc = gy_synchronize(c);
+  if (rc && rc != GY_ERR_RANGE) return rc;
+  Ctrl h{};
+  HIPCHECK(c, hipMemcpy(&h, c->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  const long long cap = c->d.cap;
+  const int n = c->d.np;
+  if (o->n != n) return fail(c, GY_ERR_INVALID, "download view has n=%d, context has %d", o->n, n);
+  std::vector<uint32_t> buf((size_t)NF * cap);
+  HIPCHECK(c, hipMemcpy(buf.data(), c->d.pbuf[h.parity], buf.size() * 4, hipMemcpyDeviceToHost));
+  const InstHost &H = c->ih[0];
+  const InstDesc &I = c->inst[0];
+  const double h_ = H.grid.cellsize, inv = h_ / (double)(1 << I.fbits);
+  auto U = [&](int f, int s) { return buf[(size_t)f * cap + s]; };
+  auto Fv = [&](int f, int s) { float y; uint32_t u = U(f, s); std::memcpy(&y, &u, 4); return (double)y; };
+  const int Cx = H.n[0] - 1, Cy = H.n[1] - 1;
+  for (int s = 0; s < n; s++) {
+    const int i = (int)U(F_PID, s);
+    if (i < 0 || i >= n) return fail(c, GY_ERR_STATE, "corrupt particle id at slot %d", s);
+    const uint32_t tag = U(F_TAG, s);
+    const int m = tag_mat(tag);
+    double x[3];
+    for (int a = 0; a < 3; a++) x[a] = (double)(int32_t)U(F_QX + a, s) * inv;
+    for (int a = 0; a < 3; a++) {
+      if (o->pos) o->pos[3 * i + a] = x[a];
+      if (o->pos0) o->pos0[3 * i + a] = H.pos0[3 * i + a];
+      if (o->displacement) o->displacement[3 * i + a] = x[a] - H.pos0[3 * i + a];
+      if (o->vel) o->vel[3 * i + a] = Fv(F_VX + a, s);
+    }
+    const bool cp = H.carry_plastic[m], cd = H.carry_damage[m];
+    for (int k = 0; k < 6; k++) {
+      if (o->stress) o->stress[6 * i + k] = Fv(F_S0 + k, s);
+      if (o->strain) o->strain[6 * i + k] = Fv(F_E0 + k, s);
+      if (o->plastic_strain) o->plastic_strain[6 * i + k] = cp ? Fv(F_P0 + k, s) : 0.0;
+    }
+    const double rho = Fv(F_RHO, s), mass = Fv(F_MASS, s);
+    if (o->acc_plastic_strain) o->acc_plastic_strain[i] = cp ? Fv(F_EPB, s) : 0.0;
+    if (o->density) o->density[i] = rho;
+    if (o->mass) o->mass[i] = mass;
+    if (o->volume) o->volume[i] = mass / rho;
+    if (o->damage) o->damage[i] = cd ? Fv(F_DMG, s) : 0.0;
+    if (o->smax) o->smax[i] = cd ? Fv(F_SMAX, s) : 0.0;
+    if (o->material) o->material[i] = m;
+    if (o->obj_id) o->obj_id[i] = tag_obj(tag);
+    if (o->hold) o->hold[i] = (uint8_t)tag_hold(tag);
+    if (o->bc_type) o->bc_type[i] = tag_bct(tag);
+    if (o->cell_id) {  // TimeIntegration.cpp:110-115
+      const int ix = (int)((x[0] - 0.25 * h_) / h_), iy = (int)((x[1] - 0.25 * h_) / h_), iz = (int)((x[2] - 0.25 * h_) / h_);
+      o->cell_id[i] = ix + iy * Cx + iz * Cx * Cy;
+    }
+  }
+  return rc;
+}
+
+int gy_audit(gy_ctx *c, gy_audit_out *out) {
+  if (!c || !out) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  timed(c, K_AUDIT, [&] { launch_audit(c->d, c->stream); });
+  int rc = gy_synchronize(c);
+  if (rc && rc != GY_ERR_RANGE) return rc;
+  const int nb = (c->d.np + 255) / 256;
+  double r[8];
+  HIPCHECK(c, hipMemcpy(r, c->d.audit + (size_t)nb * 8, sizeof r, hipMemcpyDeviceToHost));
+  Ctrl h{};
+  HIPCHECK(c, hipMemcpy(&h, c->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  out->mass = r[0];
+  out->momentum[0] = r[1];
+  out->momentum[1] = r[2];
+  out->momentum[2] = r[3];
+  out->kinetic_energy = r[4];
+  out->strain_energy_proxy = r[5];
+  out->nonfinite = (int64_t)llround(r[6]);
+  out->out_of_range = h.oor;
+  return rc;
+}
+
+int gy_debug_binning(gy_ctx *c, int32_t *keys, int32_t *counts, int32_t *offsets, int32_t *order,
+                     int64_t *nkeys, int64_t *nslots) {
+  if (!c) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  if (!c->acc_valid) {  // bin the uploaded state
+    launch_set_ctrl(c->d, c->host_step, 0.0, 0, c->stream);
+    enqueue_init(c);
+    c->acc_valid = true;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const DevPtrs &d = c->d;
+  if (nkeys) *nkeys = d.nkeys;
+  if (nslots) *nslots = d.np;
+  Ctrl h{};
+  HIPCHECK(c, hipMemcpy(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  if (keys) HIPCHECK(c, hipMemcpy(keys, d.key[h.parity], (size_t)d.np * 4, hipMemcpyDeviceToHost));
+  std::vector<int> off((size_t)d.nkeys + 1);
+  HIPCHECK(c, hipMemcpy(off.data(), d.offsets, off.size() * 4, hipMemcpyDeviceToHost));
+  if (offsets) std::memcpy(offsets, off.data(), (size_t)d.nkeys * 4);
+  if (counts)
+    for (int k = 0; k < d.nkeys; k++) counts[k] = off[k + 1] - off[k];
+  if (order) {
+    int *dord = nullptr;
+    HIPCHECK(c, hipMalloc(&dord, (size_t)d.np * 4));
+    launch_stable_order(d, dord, c->stream);
+    HIPCHECK(c, hipMemcpyAsync(order, dord, (size_t)d.np * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dord);
+  }
+  return GY_OK;
+}
+
+int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *counts, int32_t *offsets, int32_t *order) {
+  if (!keys || n <= 0 || nkeys <= 0 || n > 0x7fffffff || nkeys > 0x7ffffff0) return GY_ERR_INVALID;
+  for (int64_t i = 0; i < n; i++)
+    if (keys[i] < 0 || keys[i] >= nkeys) return GY_ERR_INVALID;
+  gy_ctx tmp;
+  if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess) return GY_ERR_HIP;
+  DevPtrs &d = tmp.d;
+  d.np = (int)n;
+  d.nkeys = (int)nkeys;
+  d.nblocks_scan = (int)((nkeys + scan_items_per_block() - 1) / scan_items_per_block());
+  int rc = GY_OK;
+  if ((rc = dalloc(&tmp, &d.key[0], (size_t)n)) || (rc = dalloc(&tmp, &d.rank, (size_t)n)) ||
+      (rc = dalloc(&tmp, &d.tmp, (size_t)n)) || (rc = dalloc(&tmp, &d.count, (size_t)nkeys)) ||
+      (rc = dalloc(&tmp, &d.offsets, (size_t)nkeys + 1)) || (rc = dalloc(&tmp, &d.blocksum, (size_t)d.nblocks_scan)) ||
+      (rc = dalloc(&tmp, &d.ctrl, 1))) {
+    rc = GY_ERR_HIP;
+  } else {
+    int *dord = nullptr;
+    if (hipMalloc(&dord, (size_t)n * 4) != hipSuccess ||
+        hipMemcpyAsync(d.key[0], keys, (size_t)n * 4, hipMemcpyHostToDevice, tmp.stream) != hipSuccess) {
+      rc = GY_ERR_HIP;
+    } else {
+      launch_hist(d, tmp.stream);
+      launch_scan_partial(d, tmp.stream);
+      launch_scan_final(d, tmp.stream);
+      launch_place(d, tmp.stream);
+      launch_stable_order(d, dord, tmp.stream);
+      std::vector<int> off((size_t)nkeys + 1);
+      if (hipMemcpyAsync(off.data(), d.offsets, off.size() * 4, hipMemcpyDeviceToHost, tmp.stream) != hipSuccess ||
+          (order && hipMemcpyAsync(order, dord, (size_t)n * 4, hipMemcpyDeviceToHost, tmp.stream) != hipSuccess) ||
+          hipStreamSynchronize(tmp.stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+        rc = GY_ERR_HIP;
+      } else {
+        if (offsets) std::memcpy(offsets, off.data(), (size_t)nkeys * 4);
+        if (counts)
+          for (int64_t k = 0; k < nkeys; k++) counts[k] = off[k + 1] - off[k];
+      }
+    }
+    dfree(dord);
+  }
+  dfree(d.key[0]); dfree(d.rank); dfree(d.tmp); dfree(d.count); dfree(d.offsets); dfree(d.blocksum); dfree(d.ctrl);
+  (void)hipStreamDestroy(tmp.stream);
+  tmp.stream = nullptr;
+  return rc;
+}
+
+int gy_set_option(gy_ctx *c, int32_t option, int64_t value) {
+  if (!c) return GY_ERR_INVALID;
+  switch (option) {
+    case GY_OPT_USE_GRAPH: c->use_graph = value != 0; if (!c->use_graph) drop_graph(c); return GY_OK;
+    case GY_OPT_PROFILE: c->profile = value != 0; return GY_OK;
+    case GY_OPT_TILE: return value == TILE ? GY_OK : fail(c, GY_ERR_UNSUPPORTED, "tile edge %lld not built", (long long)value);
+    default: return fail(c, GY_ERR_INVALID, "unknown option %d", option);
+  }
+}
+
+int gy_kernel_count(void) { return K_COUNT_; }
+const char *gy_kernel_name(int32_t i) { return (i >= 0 && i < K_COUNT_) ? kKernelNames[i] : ""; }
+
+int gy_kernel_stats(gy_ctx *c, int32_t i, int64_t *launches, double *total_ms) {
+  if (!c || i < 0 || i >= K_COUNT_) return GY_ERR_INVALID;
+  if (launches) *launches = c->launches[i];
+  if (total_ms) *total_ms = c->total_ms[i];
+  return GY_OK;
+}
+
+int gy_reset_kernel_stats(gy_ctx *c) {
+  if (!c) return GY_ERR_INVALID;
+  (void)hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  for (int i = 0; i < K_COUNT_; i++) { c->launches[i] = 0; c->total_ms[i] = 0; }
+  return GY_OK;
+}
+
+void *gy_stream(gy_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+void gy_destroy(gy_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  drop_graph(c);
+  DevPtrs &d = c->d;
+  for (int b = 0; b < 2; b++) { dfree(d.pbuf[b]); dfree(d.key[b]); }
+  dfree(d.rank); dfree(d.tmp); dfree(d.count); dfree(d.offsets); dfree(d.blocksum);
+  dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.bc_type); dfree(d.bc_val);
+  dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
+  dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

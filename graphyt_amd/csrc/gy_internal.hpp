@@ -1,0 +1,165 @@
+// gy_internal.hpp — device data layout shared by the host context and the HIP kernels.
+//
+// Particles: one 32-bit-word SoA block per state buffer, field f of slot s at
+// base[f * cap + s] (fields listed in PField). Positions are per-axis fixed point:
+// q = x / h * 2^F (int32), so a particle's cell and its offset inside the cell are exact
+// integers/floats and single-precision round-off never depends on the distance from the
+// origin. Two state buffers (A, B) alternate every step: the fused particle kernel reads
+// slot order of one binning and writes the other in cell-sorted order.
+//
+// Grid: per node 8 floats (AoS, 32 B). ACC holds P2G sums (m, px, py, pz, fx, fy, fz, -)
+// on the periodic-canonical nodes (index i folded to i-(N-3) when i >= N-3, the pair
+// identification of Grid.cpp:320-602); VEL holds the solved body-1 velocity and
+// acceleration (vx, vy, vz, ax, ay, az, m, -) on every actual node, BCs applied.
+//
+// Binning: cells are numbered tile-major (tiles of TILE^3 cells, global tile id across
+// all instances), so a counting sort by cell also groups particles by tile.
+#pragma once
+#include <cstdint>
+
+namespace gy {
+
+constexpr int TILE = 4;                    // cells per tile edge
+constexpr int TILE3 = TILE * TILE * TILE;  // cells (keys) per tile
+constexpr int BRICK = 4;                   // canonical nodes per grid-brick edge
+constexpr int G2P_R = TILE + 2;            // staged VEL region edge (nodes)
+constexpr int P2G_R = TILE + 4;            // LDS accumulator region edge (nodes), halo -1/+1
+constexpr int NODE_W = 8;                  // floats per node record
+
+enum PField : int {
+  F_QX = 0, F_QY, F_QZ,                     // fixed-point position (int32 bits)
+  F_VX, F_VY, F_VZ,                         // velocity
+  F_S0, F_S1, F_S2, F_S3, F_S4, F_S5,       // stress xx yy zz xy yz xz
+  F_E0, F_E1, F_E2, F_E3, F_E4, F_E5,       // total strain
+  F_P0, F_P1, F_P2, F_P3, F_P4, F_P5,       // plastic strain
+  F_EPB,                                    // accumulated plastic strain
+  F_RHO,                                    // density
+  F_MASS,                                   // mass
+  F_DMG,                                    // damage
+  F_SMAX,                                   // max principal stress
+  F_TAG,                                    // packed: mat(8) | hold(1) | bc type(7) | obj(16)
+  F_PID,                                    // caller's particle index
+  NF
+};
+
+__host__ __device__ inline uint32_t make_tag(int mat, int hold, int bct, int obj) {
+  return (uint32_t)(mat & 0xff) | ((uint32_t)(hold & 1) << 8) | ((uint32_t)(bct & 0x7f) << 9) |
+         ((uint32_t)(obj & 0xffff) << 16);
+}
+__host__ __device__ inline int tag_mat(uint32_t t) { return (int)(t & 0xff); }
+__host__ __device__ inline int tag_hold(uint32_t t) { return (int)((t >> 8) & 1); }
+__host__ __device__ inline int tag_bct(uint32_t t) { return (int)((t >> 9) & 0x7f); }
+__host__ __device__ inline int tag_obj(uint32_t t) { return (int)(t >> 16); }
+
+// Per-instance description (one MPM problem; a context holds one or many).
+struct InstDesc {
+  int n[3];          // nodes per axis (Nodes.cpp:71-73)
+  int cper[3];       // canonical extent per axis = n - 3
+  int nt[3];         // tiles per axis
+  int nb[3];         // canonical bricks per axis
+  int tile_base;     // first global tile
+  int brick_base;    // first global brick
+  long long node_base;  // first global node
+  int fbits;         // fixed-point fraction bits F
+  int lpq;           // lp (= psep/2) in fixed units
+  int oneq;          // h in fixed units (2^F)
+  int wrapq[3];      // particle wrap period L-2h in fixed units (TimeIntegration.cpp:81)
+  int mat_base;      // first entry in the material table
+  int nmat;          // materials of this instance
+  int pstart;        // first particle slot of this instance (slots stay grouped)
+  int pcount;        // particles of this instance
+  float h, inv_h;
+  float lp;          // lp / h
+  float inv_lp;      // h / lp
+  float dt;
+  float qscale;      // dt * 2^F / h: fixed-point units per (m/s) of grid velocity
+  float gravity[3];
+  float damping;
+  float inv_2F;      // 2^-F
+  float pad1;
+};
+
+// Per-material constants, derived on the host in FP64 (ConstitutiveModels.cpp,
+// DamageModels.cpp) and stored in FP32 except for the Grady-Kipp block.
+struct MatDev {
+  int model;        // 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager
+  int rigid;
+  int damage;       // isDamage (also switches the Drucker-Prager criterion)
+  int gk;           // Grady-Kipp damage update runs (isDamage && damageModel == 1)
+  int apex_guard;
+  int carry_plastic;  // plastic strain fields are live for this material
+  int carry_damage;   // damage / smax fields are live for this material
+  float K, G, visc, sy;
+  float c, ecr, eta, xi, eta_bar;
+  float c_over_ecr, c_floor;   // c/eps_cr, c/10
+  float xi_over_eta, xi_over_etab;
+  double gk_K43;      // K + 4G/3
+  double gk_alpha13;  // alpha^(1/3)
+  double gk_m, gk_k, gk_m3;  // m, k, (m+3)/3
+};
+
+// Step control read by every kernel of a step (updated on the device, so a captured
+// graph can be replayed for many steps).
+struct Ctrl {
+  int step;      // reference step number (half kick at step == 1, Grid.cpp:126)
+  int parity;    // which state buffer is the source this step
+  int status;    // sticky error bits (1 = particle stencil left the grid)
+  int oor;       // count of out-of-range particles (clamped)
+};
+
+// All device pointers of a context (passed by value to kernels).
+struct DevPtrs {
+  uint32_t *pbuf[2];     // particle state buffers, NF * cap words each
+  int *key[2];           // cell key per slot, per buffer
+  int *rank;             // provisional rank within the key (histogram atomic)
+  int *tmp;              // key-grouped slot list of the current binning
+  int *count;            // per-key histogram (nkeys)
+  int *offsets;          // per-key exclusive offsets (nkeys + 1)
+  int *blocksum;         // scan partials
+  float *acc;            // NODE_W per global node
+  float *vel;            // NODE_W per global node
+  uint8_t *brick_flag;   // per global canonical brick
+  uint8_t *bc_type;      // per global node
+  float *bc_val;         // 2 per global node
+  float *pbc_val;        // 4 per particle (indexed by pid)
+  int *tile_inst;        // global tile -> instance
+  int *brick_inst;       // global brick -> instance
+  InstDesc *inst;
+  MatDev *mats;
+  Ctrl *ctrl;
+  double *t_last;        // per instance: time of the previous step (sine BCs)
+  double *t_cur;         // per instance: time of the current step
+  double *audit;         // audit partials
+  long long cap;         // slots per buffer
+  int np;                // live particles (all instances)
+  int ninst;
+  int nkeys;             // total keys = total tiles * TILE3
+  int ntiles;
+  int nbricks;
+  int nblocks_scan;
+};
+
+// Kernel identifiers for per-kernel timing.
+enum KernelId : int {
+  K_KEYS = 0, K_SCAN_PARTIAL, K_SCAN_FINAL, K_PLACE, K_P2G, K_GRID, K_G2P2G, K_ADVANCE,
+  K_AUDIT, K_ORDER, K_COUNT_
+};
+
+// Host-side launchers (defined in gy_kernels.hip). Every kernel reads the step number and
+// the buffer parity from Ctrl, so a captured sequence replays correctly step after step.
+// Order of one step: grid, g2p2g, advance, scan_partial, scan_final, place.
+void launch_set_ctrl(const DevPtrs &d, int step, double t, int reset, void *stream);
+void launch_keys(const DevPtrs &d, void *stream);
+void launch_hist(const DevPtrs &d, void *stream);  // histogram of key[0] (standalone sort)
+void launch_scan_partial(const DevPtrs &d, void *stream);
+void launch_scan_final(const DevPtrs &d, void *stream);
+void launch_place(const DevPtrs &d, void *stream);
+void launch_p2g_only(const DevPtrs &d, void *stream);
+void launch_grid(const DevPtrs &d, void *stream);
+void launch_g2p2g(const DevPtrs &d, void *stream);
+void launch_advance(const DevPtrs &d, void *stream);
+void launch_audit(const DevPtrs &d, void *stream);
+void launch_stable_order(const DevPtrs &d, int *order, void *stream);
+int scan_items_per_block();
+
+}  // namespace gy

@@ -1,0 +1,963 @@
+// gy_kernels.hip — HIP kernels (gfx950) of the MPM time step.
+//
+// One step is six launches on one stream:
+//   k_grid      nodal solve on the active canonical bricks: v = p/m, a = f/m + g,
+//               body-1 kick (half at step 1), BCs, periodic fan-out to every mirror node,
+//               and the P2G accumulators cleared for the next step
+//               (Grid.cpp:9-245, 310-603, 862-1094)
+//   k_g2p2g     per cell tile: G2P from LDS-staged node velocities, particle update,
+//               constitutive/damage models, periodic wrap, new cell key + histogram, and
+//               P2G of the updated particle into an LDS tile with a one-node halo, flushed
+//               with a few global FP32 atomics (Extrapolation.cpp:31-367,
+//               TimeIntegration.cpp:11-296, ConstitutiveModels.cpp, DamageModels.cpp)
+//   k_advance   step/parity/time bookkeeping on the device
+//   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets)
+//   k_place     key-grouped slot list; the stable within-cell order (ascending slot) is
+//               resolved where the list is consumed
+// Binning replaces sortParticlesByCellID_CPU (Extrapolation.cpp:370-503) and runs every step.
+#include <hip/hip_runtime.h>
+
+#include "gy_internal.hpp"
+
+namespace gy {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
+constexpr int MAXMAT = 16;
+constexpr int NODES_G2P = G2P_R * G2P_R * G2P_R;
+constexpr int NODES_P2G = P2G_R * P2G_R * P2G_R;
+
+__device__ __forceinline__ float f32(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
+
+// Contiguous tile ranges per XCD: blocks are dispatched round-robin over the 8 XCDs, so
+// block b goes to XCD b % 8; map it to a tile inside that XCD's contiguous chunk, which
+// keeps neighbouring tiles (shared halo nodes) in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
+// Stencil base cell of a fixed-point coordinate, clamped to the grid (getGIMPNodes,
+// Extrapolation.cpp:658-660: int((x - psep/2 - x0)/h)); `bad` is set when clamped.
+__device__ __forceinline__ int stencil_base(int q, int lpq, int F, int maxb, bool &bad) {
+  int b = (q - lpq) >> F;
+  if (b < 0) { b = 0; bad = true; }
+  if (b > maxb) { b = maxb; bad = true; }
+  return b;
+}
+
+__device__ __forceinline__ int key_of(const InstDesc &I, const int b[3]) {
+  const int t = I.tile_base + (b[0] / TILE) + I.nt[0] * ((b[1] / TILE) + I.nt[1] * (b[2] / TILE));
+  return t * TILE3 + (b[0] % TILE) + TILE * ((b[1] % TILE) + TILE * (b[2] % TILE));
+}
+
+// GIMP weights of the three stencil nodes along one axis (Extrapolation.cpp:556-647 with
+// h = 1), for a particle at offset u in [lp, 1+lp) from the base node and lp <= 1/2.
+// S[i] = S(d_i), G[i] = dS/dd at d_i = i - u; the physical gradient dN/dx_p is -G/h.
+struct Axis {
+  float w[3], g[3];
+};
+__device__ __forceinline__ Axis gimp3(float u, float lp, float inv_lp, float inv_h) {
+  Axis A;
+  const float d0 = -u, d1 = 1.0f - u, d2 = 2.0f - u;
+  const float e0 = 1.0f + lp + d0;  // distance past -(1+lp)
+  // node 0: d0 in (-1-lp, -lp]
+  if (d0 <= -1.0f + lp) { A.w[0] = e0 * e0 * (0.25f * inv_lp); } else { A.w[0] = 1.0f + d0; }
+  const float G0 = (d0 < -1.0f + lp) ? e0 * (0.5f * inv_lp) : 1.0f;
+  // node 1: d1 in (-lp, 1-lp]
+  if (d1 <= lp) { A.w[1] = 1.0f - (d1 * d1 + lp * lp) * (0.5f * inv_lp); } else { A.w[1] = 1.0f - d1; }
+  const float G1 = (d1 < lp) ? -d1 * inv_lp : -1.0f;
+  // node 2: d2 in (1-lp, 2-lp]
+  const float e2 = 1.0f + lp - d2;
+  float G2;
+  if (d2 <= 1.0f + lp) { A.w[2] = e2 * e2 * (0.25f * inv_lp); G2 = -e2 * (0.5f * inv_lp); }
+  else { A.w[2] = 0.0f; G2 = 0.0f; }
+  A.g[0] = -G0 * inv_h;
+  A.g[1] = -G1 * inv_h;
+  A.g[2] = -G2 * inv_h;
+  return A;
+}
+
+// ---------------------------------------------------------------------------------------
+// Constitutive models (FP32), following ConstitutiveModels.cpp operation by operation.
+
+struct PState {
+  float v[3], s[6], e[6], p[6], epb, rho, mass, dmg, smax;
+};
+
+// elastic, ConstitutiveModels.cpp:3-30
+__device__ __forceinline__ void cm_elastic(const MatDev &M, const float er[6], float s[6], float dt) {
+  const float vol = 0.3333f * (er[0] + er[1] + er[2]);
+  const float G2 = 2.0f * M.G, K3 = 3.0f * M.K;
+  s[0] += (G2 * (er[0] - vol) + K3 * vol) * dt;
+  s[1] += (G2 * (er[1] - vol) + K3 * vol) * dt;
+  s[2] += (G2 * (er[2] - vol) + K3 * vol) * dt;
+  s[3] += (G2 * er[3]) * dt;
+  s[4] += (G2 * er[4]) * dt;
+  s[5] += (G2 * er[5]) * dt;
+}
+
+// newtonianLiquid, ConstitutiveModels.cpp:33-64
+__device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6], const float e[6],
+                                             float s[6], float dt) {
+  const float vol = 0.33333f * (er[0] + er[1] + er[2]);
+  const float P = M.K * (e[0] + e[1] + e[2]);
+  const float c = M.visc * (2.0f - (2.0f / 3.0f));
+  s[0] = P + c * (er[0] - vol) * dt;
+  s[1] = P + c * (er[1] - vol) * dt;
+  s[2] = P + c * (er[2] - vol) * dt;
+  s[3] = 2.0f * M.visc * er[3] * dt;
+  s[4] = 2.0f * M.visc * er[4] * dt;
+  s[5] = 2.0f * M.visc * er[5] * dt;
+}
+
+// perfectPlasticity, ConstitutiveModels.cpp:131-234
+__device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], float s[6], float p[6],
+                                           float &epb, float dt) {
+  const float G = M.G, K = M.K;
+  const float vol = 0.33f * (er[0] + er[1] + er[2]);
+  float tr[6];
+  tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
+  tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
+  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
+  tr[3] = s[3] + (2.0f * G * er[3]) * dt;
+  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
+  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
+  const float a01 = tr[0] - tr[1], a12 = tr[1] - tr[2], a20 = tr[2] - tr[0];
+  const float q = sqrtf(0.5f * a01 * a01 + 0.5f * a12 * a12 + 0.5f * a20 * a20 + tr[3] * tr[3] +
+                        tr[4] * tr[4] + tr[5] * tr[5]);
+  if (!(q > M.sy)) {
+    for (int k = 0; k < 6; k++) s[k] = tr[k];
+    return;
+  }
+  const float dL = (q - M.sy) / (3.0f * G);
+  const float f = dL * (0.5f / q);
+  float pr[6];
+  pr[0] = f * (2.0f * tr[0] - tr[1] - tr[2]);
+  pr[1] = f * (2.0f * tr[1] - tr[2] - tr[0]);
+  pr[2] = f * (2.0f * tr[2] - tr[0] - tr[1]);
+  pr[3] = f * (6.0f * tr[3]);
+  pr[4] = f * (6.0f * tr[4]);
+  pr[5] = f * (6.0f * tr[5]);
+  const float b01 = pr[0] - pr[1], b12 = pr[1] - pr[2], b20 = pr[2] - pr[0];
+  epb += dt * sqrtf(0.5f * b01 * b01 + 0.5f * b12 * b12 + 0.5f * b20 * b20 + pr[3] * pr[3] +
+                    pr[4] * pr[4] + pr[5] * pr[5]);
+  const float inv_dt = 1.0f / dt;
+  float el[6];
+  for (int k = 0; k < 6; k++) {
+    p[k] += pr[k];
+    el[k] = er[k] - pr[k] * inv_dt;
+  }
+  s[0] += (2.0f * G * (el[0] - vol) + 3.0f * K * vol) * dt;
+  s[1] += (2.0f * G * (el[1] - vol) + 3.0f * K * vol) * dt;
+  s[2] += (2.0f * G * (el[2] - vol) + 3.0f * K * vol) * dt;
+  s[3] += (2.0f * G * el[3]) * dt;
+  s[4] += (2.0f * G * el[4]) * dt;
+  s[5] += (2.0f * G * el[5]) * dt;
+}
+
+// druckerPrager_Damage, ConstitutiveModels.cpp:238-428. The two Newton loops solve
+// piecewise-linear equations; the reference iterates to |F/c| <= 1e-8 or 201 passes, this
+// FP32 form iterates to its fixed point (at most 16 passes) — the same root to rounding.
+__device__ __forceinline__ void cm_drucker_prager(const MatDev &M, const float er[6], float s[6],
+                                                  float p[6], float &epb, float dt) {
+  const float G = M.G, K = M.K;
+  const float vol = 0.3333f * (er[0] + er[1] + er[2]);
+  float tr[6];
+  tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
+  tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
+  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
+  tr[3] = s[3] + (2.0f * G * er[3]) * dt;
+  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
+  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
+  const float I1 = (1.0f / 3.0f) * (tr[0] + tr[1] + tr[2]);
+  const float sxx = tr[0] - I1, syy = tr[1] - I1, szz = tr[2] - I1;
+  const float sxy = tr[3], syz = tr[4], sxz = tr[5];
+  const float J2 = 0.5f * (sxx * sxx + syy * syy + szz * szz + 2.0f * sxy * sxy + 2.0f * syz * syz +
+                           2.0f * sxz * sxz);
+  const float sJ2 = sqrtf(J2);
+  const float F = sJ2 + M.eta * I1 - M.xi * M.c;
+  const bool yield = M.damage ? ((F > 0.0f) && (I1 < 0.0f)) : (F > 0.0f);
+  if (!yield) {
+    for (int k = 0; k < 6; k++) s[k] = tr[k];
+    return;
+  }
+  const float ep0 = epb;
+  float dL = 0.0f;
+  const float slope0 = -G - K * M.eta * M.eta_bar;
+  for (int it = 0; it < 16; it++) {
+    float cv = M.c - (ep0 + M.xi * dL) * M.c_over_ecr;
+    float dcv = -M.c_over_ecr;
+    if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
+    const float Fp = sJ2 - G * dL + M.eta * (I1 - M.eta_bar * K * dL) - M.xi * cv;
+    const float dFp = slope0 - M.xi * M.xi * dcv;
+    const float dLn = dL - Fp / dFp;
+    const bool done = (dLn == dL) || !(dLn == dLn);
+    dL = dLn;
+    if (done) break;
+  }
+  if ((sJ2 - G * dL < 0.0f) && !M.damage) {
+    // apex return (ConstitutiveModels.cpp:347-395)
+    float de = 0.0f;
+    for (int it = 0; it < 16; it++) {
+      float cv = M.c - (ep0 + M.xi_over_eta * de) * M.c_over_ecr;
+      float dcv = -M.c_over_ecr;
+      if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
+      float Gc = cv * M.xi_over_etab, dGc = M.xi_over_eta * dcv * M.xi_over_etab;
+      if (M.apex_guard) {
+        if (cv == 0.0f) Gc = 0.0f;
+        if (dcv == 0.0f) dGc = 0.0f;
+      }
+      const float Gp = Gc - I1 + K * de;
+      const float den = de - Gp / (dGc + K);
+      const bool done = (den == de) || !(den == den);
+      de = den;
+      if (done) break;
+    }
+    if (de < 0.0f) return;  // deps < 0: stress left as it was (ConstitutiveModels.cpp:376-378)
+    const float sm = I1 - K * de;
+    s[0] = sm; s[1] = sm; s[2] = sm; s[3] = 0.0f; s[4] = 0.0f; s[5] = 0.0f;
+    epb += de * M.xi_over_eta;
+    const float pv = de * (M.eta_bar / 3.0f);
+    p[0] += pv; p[1] += pv; p[2] += pv;
+    return;
+  }
+  const float r = 1.0f - G * dL / sJ2;
+  const float sm = I1 - K * M.eta_bar * dL;
+  s[0] = r * sxx + sm;
+  s[1] = r * syy + sm;
+  s[2] = r * szz + sm;
+  s[3] = r * sxy;
+  s[4] = r * syz;
+  s[5] = r * sxz;
+  epb += dL * M.xi;
+  const float hj = 0.5f / J2, pv = M.eta_bar / 3.0f;
+  p[0] += dL * (hj * sxx + pv);
+  p[1] += dL * (hj * syy + pv);
+  p[2] += dL * (hj * szz + pv);
+  p[3] += dL * hj * sxy;
+  p[4] += dL * hj * syz;
+  p[5] += dL * hj * sxz;
+}
+
+// Symmetric 3x3 eigen-decomposition in FP64 by cyclic Jacobi rotations (Golub & Van Loan,
+// Alg. 8.5.3); eigenvectors in the columns of V.
+__device__ void jacobi3(double a[3][3], double V[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; sweep++) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-36 * dg || off == 0.0) break;
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+      const int p = (pq == 2) ? 1 : 0, q = (pq == 0) ? 1 : 2;
+      if (a[p][q] == 0.0) continue;
+      const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+      for (int k = 0; k < 3; k++) {
+        const double akp = a[k][p], akq = a[k][q];
+        a[k][p] = c * akp - sn * akq;
+        a[k][q] = sn * akp + c * akq;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double apk = a[p][k], aqk = a[q][k];
+        a[p][k] = c * apk - sn * aqk;
+        a[q][k] = sn * apk + c * aqk;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double vkp = V[k][p], vkq = V[k][q];
+        V[k][p] = c * vkp - sn * vkq;
+        V[k][q] = sn * vkp + c * vkq;
+      }
+    }
+  }
+}
+
+// gradyKippDamage 3D branch (DamageModels.cpp:71-150), the whole block in FP64.
+__device__ void gk_damage(const MatDev &M, float s[6], float &D, float &smax, float mass, float rho,
+                          float dt) {
+  double a[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+  const double check = fabs(a[0][0] + a[1][1] + a[2][2] + a[0][1] + a[1][2] + a[0][2]);
+  if (!(check > 1e-3)) return;
+  double V[3][3];
+  jacobi3(a, V);
+  double ev[3] = {a[0][0], a[1][1], a[2][2]};
+  const double smx = fmax(ev[0], fmax(ev[1], fmax(ev[2], 0.0)));
+  smax = (float)smx;
+  if (smx > 1e-3) {
+    const double eff = smx / M.gk_K43;
+    const double vol = (double)mass / (double)rho;
+    const double emin = pow(vol * M.gk_k, -1.0 / M.gk_m);
+    if (eff > emin) {
+      const double rate = M.gk_m3 * M.gk_alpha13 * pow(eff, M.gk_m / 3.0);
+      double Dn = pow(pow((double)D, 1.0 / 3.0) + (double)dt * rate, 3.0);
+      if (Dn > 1.0) Dn = 1.0;
+      D = (float)Dn;
+      for (int k = 0; k < 3; k++)
+        if (ev[k] > 0.0) ev[k] = (1.0 - Dn) * ev[k];
+    }
+  }
+  double o[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < 3; c++) {
+    const double n = sqrt(V[0][c] * V[0][c] + V[1][c] * V[1][c] + V[2][c] * V[2][c]);
+    const double x = V[0][c] / n, y = V[1][c] / n, z = V[2][c] / n, l = ev[c];
+    o[0] += x * l * x; o[1] += y * l * y; o[2] += z * l * z;
+    o[3] += x * l * y; o[4] += y * l * z; o[5] += x * l * z;
+  }
+  for (int k = 0; k < 6; k++) s[k] = (float)o[k];
+}
+
+// Particle BC handlers, TimeIntegration.cpp:177-296 (phase is ignored by the reference).
+__device__ __forceinline__ void particle_bc(int type, const float *bv, float t, float v[3], float a[3],
+                                            float vg[3], float s[6], float mass) {
+  switch (type) {
+    case 33: v[0] = bv[0]; break;
+    case 34: v[0] = 0.0f; v[1] = bv[0]; v[2] = 0.0f; break;
+    case 35: v[0] = 0.0f; v[1] = 0.0f; v[2] = bv[0]; break;
+    case 40: v[0] = bv[0]; v[1] = bv[1]; v[2] = 0.0f; break;
+    case 65: v[0] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t); break;
+    case 66: v[1] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t); vg[1] = 0.0f; a[1] = 0.0f; break;
+    case 67:
+      s[1] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t);
+      v[0] = v[1] = v[2] = 0.0f; vg[0] = vg[1] = vg[2] = 0.0f; break;
+    case 37: a[0] += bv[0] / mass; break;
+    case 38: a[0] = 0.0f; a[1] += bv[0] / mass; break;
+    case 39: a[2] += bv[0] / mass; break;
+    case 43: a[0] += bv[0] / mass; a[1] += bv[1] / mass; break;
+    case 63: s[0] = bv[0]; s[1] = bv[1]; break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+__global__ void k_set_ctrl(DevPtrs d, int step, double t, int reset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ninst) d.t_cur[i] = t;
+  if (i == 0) {
+    d.ctrl->step = step;
+    if (reset) { d.ctrl->parity = 0; d.ctrl->status = 0; d.ctrl->oor = 0; }
+  }
+}
+
+__global__ void k_advance(DevPtrs d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ninst) {
+    d.t_last[i] = d.t_cur[i];
+    d.t_cur[i] += (double)d.inst[i].dt;
+  }
+  if (i == 0) {
+    d.ctrl->step += 1;
+    d.ctrl->parity ^= 1;
+  }
+}
+
+__device__ __forceinline__ int inst_of_slot(const DevPtrs &d, int s) {
+  int lo = 0, hi = d.ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.inst[mid].pstart <= s) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Keys + histogram of the current buffer (first step after an upload).
+__global__ __launch_bounds__(WG) void k_keys(DevPtrs d) {
+  const int s = blockIdx.x * WG + threadIdx.x;
+  if (s >= d.np) return;
+  const int par = d.ctrl->parity;
+  const uint32_t *A = d.pbuf[par];
+  const InstDesc &I = d.inst[inst_of_slot(d, s)];
+  bool bad = false;
+  int b[3];
+  for (int a = 0; a < 3; a++)
+    b[a] = stencil_base((int)A[(F_QX + a) * d.cap + s], I.lpq, I.fbits, I.n[a] - 3, bad);
+  const int k = key_of(I, b);
+  d.key[par][s] = k;
+  d.rank[s] = atomicAdd(&d.count[k], 1);
+  if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
+}
+
+// Block-wide exclusive scan of one int per thread (wave64 shuffles + LDS).
+__device__ __forceinline__ int block_exclusive_scan(int v, int &total) {
+  __shared__ int wsum[WG / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < WG / 64; k++) {
+    const int t = wsum[k];
+    if (k < w) before += t;
+    all += t;
+  }
+  __syncthreads();
+  total = all;
+  return before + x - v;
+}
+
+__device__ __forceinline__ int block_sum(int v) {
+  int tot;
+  block_exclusive_scan(v, tot);
+  return tot;
+}
+
+__global__ __launch_bounds__(WG) void k_hist(DevPtrs d) {
+  const int s = blockIdx.x * WG + threadIdx.x;
+  if (s >= d.np) return;
+  d.rank[s] = atomicAdd(&d.count[d.key[0][s]], 1);
+}
+
+__global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d) {
+  const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    const long long k = base + j;
+    if (k < d.nkeys) s += d.count[k];
+  }
+  const int tot = block_sum(s);
+  if (threadIdx.x == 0) d.blocksum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
+  // prefix of all earlier blocks
+  int pre = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += WG) pre += d.blocksum[j];
+  pre = block_sum(pre);
+  const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+  int c[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    const long long k = base + j;
+    c[j] = (k < d.nkeys) ? d.count[k] : 0;
+    s += c[j];
+  }
+  int tot;
+  int run = pre + block_exclusive_scan(s, tot);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; j++) {
+    const long long k = base + j;
+    if (k < d.nkeys) {
+      d.offsets[k] = run;
+      d.count[k] = 0;
+    }
+    run += c[j];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) d.offsets[d.nkeys] = pre + tot;
+}
+
+__global__ __launch_bounds__(WG) void k_place(DevPtrs d) {
+  const int s = blockIdx.x * WG + threadIdx.x;
+  if (s >= d.np) return;
+  const int k = d.key[d.ctrl->parity][s];
+  d.tmp[d.offsets[k] + d.rank[s]] = s;
+}
+
+// Stable position of the slot listed at sorted index i: the key-grouped list holds each
+// cell's slots in arbitrary order; ascending slot order inside the cell is the stable
+// counting-sort order.
+__device__ __forceinline__ int stable_pos(const DevPtrs &d, const int *keys, int i, int slot) {
+  const int k = keys[slot];
+  const int lo = d.offsets[k], hi = d.offsets[k + 1];
+  int r = 0;
+  for (int j = lo; j < hi; j++) r += (d.tmp[j] < slot) ? 1 : 0;
+  return lo + r;
+}
+
+__global__ __launch_bounds__(WG) void k_stable_order(DevPtrs d, int *order) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= d.np) return;
+  const int slot = d.tmp[i];
+  order[stable_pos(d, d.key[d.ctrl->parity], i, slot)] = slot;
+}
+
+// ---------------------------------------------------------------------------------------
+// Grid update on the active canonical bricks (one wave per 4x4x4 brick).
+__global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gb = blockIdx.x * (WG / 64) + wave;
+  if (gb >= d.nbricks) return;
+  if (!d.brick_flag[gb]) return;
+  const InstDesc &I = d.inst[d.brick_inst[gb]];
+  const int lb = gb - I.brick_base;
+  const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
+  const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
+  const int step = d.ctrl->step;
+  if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
+    const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
+    float4 *ap = reinterpret_cast<float4 *>(d.acc + ci * NODE_W);
+    const float4 a0 = ap[0], a1 = ap[1];
+    ap[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ap[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float m = a0.x;
+    float v[3] = {0.f, 0.f, 0.f}, acc[3] = {0.f, 0.f, 0.f};
+    if (m != 0.0f) {
+      // Grid.cpp:14-16, 220-243: v = p/m; a = f/m (gravity enters as m*g, the sum of the
+      // per-pair N*m_p*g terms of Extrapolation.cpp:226-228); v += a*dt (0.5*dt at step 1)
+      const float kick = (step == 1) ? 0.5f * I.dt : I.dt;
+      const float p[3] = {a0.y, a0.z, a0.w};
+      const float f[3] = {a1.x + m * I.gravity[0], a1.y + m * I.gravity[1], a1.z + m * I.gravity[2]};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        acc[k] = f[k] / m;
+        v[k] = p[k] / m + acc[k] * kick;
+      }
+    }
+    // fan out to the actual nodes of the periodic class (Grid.cpp:320-602) with each
+    // node's own BC (Grid.cpp:862-1031)
+#pragma unroll
+    for (int mz = 0; mz < 2; mz++) {
+      if (mz && c2 >= 3) break;
+      const int n2 = c2 + mz * I.cper[2];
+#pragma unroll
+      for (int my = 0; my < 2; my++) {
+        if (my && c1 >= 3) break;
+        const int n1 = c1 + my * I.cper[1];
+#pragma unroll
+        for (int mx = 0; mx < 2; mx++) {
+          if (mx && c0 >= 3) break;
+          const int n0 = c0 + mx * I.cper[0];
+          const long long ni = I.node_base + n0 + (long long)I.n[0] * (n1 + (long long)I.n[1] * n2);
+          float vv[3] = {v[0], v[1], v[2]}, aa[3] = {acc[0], acc[1], acc[2]};
+          const int t = d.bc_type[ni];
+          if (t == 1 || t == 2 || t == 11) {
+            const float b0 = d.bc_val[2 * ni], b1 = d.bc_val[2 * ni + 1];
+            if (t == 1) { vv[0] = b0; aa[0] = 0.f; }
+            else if (t == 2) { vv[1] = b0; aa[1] = 0.f; }
+            else { vv[0] = b0; vv[1] = b1; aa[0] = 0.f; aa[1] = 0.f; }
+          }
+          float4 *vp = reinterpret_cast<float4 *>(d.vel + ni * NODE_W);
+          vp[0] = make_float4(vv[0], vv[1], vv[2], aa[0]);
+          vp[1] = make_float4(aa[1], aa[2], m, 0.f);
+        }
+      }
+    }
+  }
+  if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// Scatter one particle's P2G terms of one node: m, m*v, and the internal force
+// -V * sigma . gradN (Extrapolation.cpp:265-273).
+struct P2GTerms {
+  float m, mv[3], vs[6];  // vs = -V*sigma
+};
+
+__device__ __forceinline__ void node_terms(const P2GTerms &T, float w, float gx, float gy, float gz,
+                                           float out[7]) {
+  out[0] = w * T.m;
+  out[1] = w * T.mv[0];
+  out[2] = w * T.mv[1];
+  out[3] = w * T.mv[2];
+  out[4] = T.vs[0] * gx + T.vs[3] * gy + T.vs[5] * gz;
+  out[5] = T.vs[3] * gx + T.vs[1] * gy + T.vs[4] * gz;
+  out[6] = T.vs[5] * gx + T.vs[4] * gy + T.vs[2] * gz;
+}
+
+__device__ __forceinline__ void flag_brick(const DevPtrs &d, const InstDesc &I, const int c[3]) {
+  const int gb = I.brick_base + c[0] / BRICK + I.nb[0] * (c[1] / BRICK + I.nb[1] * (c[2] / BRICK));
+  d.brick_flag[gb] = 1;
+}
+
+__device__ __forceinline__ long long canon_index(const InstDesc &I, const int g[3], int c[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) c[a] = (g[a] >= I.cper[a]) ? g[a] - I.cper[a] : g[a];
+  return I.node_base + c[0] + (long long)I.n[0] * (c[1] + (long long)I.n[1] * c[2]);
+}
+
+// P2G of one particle straight into the global accumulators (used when its stencil lies
+// outside the tile's LDS region, e.g. after a periodic wrap).
+__device__ void p2g_global(const DevPtrs &d, const InstDesc &I, const int b[3], const Axis W[3],
+                           const P2GTerms &T) {
+#pragma unroll 1
+  for (int k = 0; k < 3; k++)
+#pragma unroll 1
+    for (int j = 0; j < 3; j++)
+#pragma unroll 1
+      for (int i = 0; i < 3; i++) {
+        const float A = W[1].w[j] * W[2].w[k];
+        const float w = W[0].w[i] * A;
+        if (w == 0.0f) continue;
+        float t[7];
+        node_terms(T, w, W[0].g[i] * A, W[0].w[i] * W[1].g[j] * W[2].w[k],
+                   W[0].w[i] * W[1].w[j] * W[2].g[k], t);
+        const int g[3] = {b[0] + i, b[1] + j, b[2] + k};
+        int c[3];
+        const long long ci = canon_index(I, g, c);
+        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&d.acc[ci * NODE_W + f], t[f]);
+        flag_brick(d, I, c);
+      }
+}
+
+// The fused particle kernel. UPDATE = false: P2G of the current state only (first step
+// after an upload). UPDATE = true: G2P + update + constitutive + P2G of the new state.
+template <bool UPDATE>
+__global__ __launch_bounds__(WG) void k_g2p2g(DevPtrs d) {
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int k0 = t * TILE3;
+  const int p0 = d.offsets[k0], p1 = d.offsets[k0 + TILE3];
+  if (p0 == p1) return;
+
+  __shared__ float s_vel[6][NODES_G2P];
+  __shared__ float s_acc[7][NODES_P2G];
+  __shared__ MatDev s_mat[MAXMAT];
+
+  const int ii = d.tile_inst[t];
+  const InstDesc I = d.inst[ii];
+  const int lt = t - I.tile_base;
+  const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
+  const int par = d.ctrl->parity;
+  const uint32_t *A = d.pbuf[par];
+  uint32_t *B = d.pbuf[par ^ 1];
+  const long long cap = d.cap;
+  const int tid = threadIdx.x;
+
+  for (int n = tid; n < 7 * NODES_P2G; n += WG) (&s_acc[0][0])[n] = 0.0f;
+  if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+  if (UPDATE) {
+    for (int n = tid; n < NODES_G2P; n += WG) {
+      const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
+      const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
+      float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+      if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
+        const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
+        const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
+        v0 = vp[0];
+        v1 = vp[1];
+      }
+      s_vel[0][n] = v0.x; s_vel[1][n] = v0.y; s_vel[2][n] = v0.z;
+      s_vel[3][n] = v0.w; s_vel[4][n] = v1.x; s_vel[5][n] = v1.y;
+    }
+  }
+  __syncthreads();
+
+  const double tl = UPDATE ? d.t_last[ii] : 0.0;
+  const int *keys_in = d.key[par];
+  int *keys_out = d.key[par ^ 1];
+
+  for (int i = p0 + tid; i < p1; i += WG) {
+    const int slot = d.tmp[i];
+    int q[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) q[a] = (int)A[(F_QX + a) * cap + slot];
+    const uint32_t tag = A[F_TAG * cap + slot];
+    const MatDev &M = s_mat[tag_mat(tag)];
+    float v[3], s[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = f32(A[(F_VX + a) * cap + slot]);
+#pragma unroll
+    for (int a = 0; a < 6; a++) s[a] = f32(A[(F_S0 + a) * cap + slot]);
+    float rho = f32(A[F_RHO * cap + slot]);
+    const float mass = f32(A[F_MASS * cap + slot]);
+    bool bad = false;
+    int b[3];
+    Axis W[3];
+
+    if (UPDATE) {
+      const int out = stable_pos(d, keys_in, i, slot);
+      float e[6], p[6] = {0, 0, 0, 0, 0, 0}, epb = 0.0f, dmg = 0.0f, smax = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 6; a++) e[a] = f32(A[(F_E0 + a) * cap + slot]);
+      const bool plastic = M.carry_plastic;
+      if (plastic) {
+#pragma unroll
+        for (int a = 0; a < 6; a++) p[a] = f32(A[(F_P0 + a) * cap + slot]);
+        epb = f32(A[F_EPB * cap + slot]);
+      }
+      if (M.carry_damage) {
+        dmg = f32(A[F_DMG * cap + slot]);
+        smax = f32(A[F_SMAX * cap + slot]);
+      }
+      // ---- G2P (Extrapolation.cpp:127-163, 288-367) ----
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+        W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+      }
+      float vg[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+      float Lx[3] = {0, 0, 0}, Ly[3] = {0, 0, 0}, Lz[3] = {0, 0, 0};  // d v_b / d x_a
+      int r0 = b[0] - o[0], r1 = b[1] - o[1], r2 = b[2] - o[2];
+      if ((unsigned)r0 >= (unsigned)TILE || (unsigned)r1 >= (unsigned)TILE || (unsigned)r2 >= (unsigned)TILE) {
+        // a particle listed under a tile it is not in: never expected; keep LDS reads in range
+        atomicOr(&d.ctrl->status, 2);
+        r0 = min(max(r0, 0), TILE - 1);
+        r1 = min(max(r1, 0), TILE - 1);
+        r2 = min(max(r2, 0), TILE - 1);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++) {
+          const float Aw = W[1].w[jj] * W[2].w[kk];
+          const float By = W[1].g[jj] * W[2].w[kk];
+          const float Bz = W[1].w[jj] * W[2].g[kk];
+          float S0[3] = {0, 0, 0}, S1[3] = {0, 0, 0}, Sa[3] = {0, 0, 0};
+          const int row = r0 + G2P_R * ((r1 + jj) + G2P_R * (r2 + kk));
+#pragma unroll
+          for (int ii2 = 0; ii2 < 3; ii2++) {
+            const int n = row + ii2;
+            const float wx = W[0].w[ii2], gx = W[0].g[ii2];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              const float vn = s_vel[c][n];
+              S0[c] += wx * vn;
+              S1[c] += gx * vn;
+              Sa[c] += wx * s_vel[3 + c][n];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            vg[c] += Aw * S0[c];
+            ap[c] += Aw * Sa[c];
+            Lx[c] += Aw * S1[c];
+            Ly[c] += By * S0[c];
+            Lz[c] += Bz * S0[c];
+          }
+        }
+      float er[6];
+      er[0] = Lx[0];
+      er[1] = Ly[1];
+      er[2] = Lz[2];
+      er[3] = 0.5f * (Lx[1] + Ly[0]);
+      er[4] = 0.5f * (Ly[2] + Lz[1]);
+      er[5] = 0.5f * (Lx[2] + Lz[0]);
+      const float rho_dot = -rho * (Lx[0] + Ly[1] + Lz[2]);
+
+      const int bct = tag_bct(tag);
+      float bv[4] = {0, 0, 0, 0};
+      const int pid = (int)A[F_PID * cap + slot];
+      if (bct) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) bv[k] = d.pbc_val[4 * (long long)pid + k];
+        particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 4 (Solver.cpp:219-223)
+      }
+      // ---- particle update (TimeIntegration.cpp:11-117) ----
+      const float dt = I.dt;
+      rho += rho_dot * dt;
+      if (!tag_hold(tag)) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          v[a] = v[a] + (ap[a] - I.damping * v[a]) * dt;
+          q[a] += __float2int_rn(vg[a] * I.qscale);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 6; a++) e[a] += er[a] * dt;
+      if (!M.rigid) {
+        if (M.model == 1) cm_elastic(M, er, s, dt);
+        else if (M.model == 2) cm_newtonian(M, er, e, s, dt);
+        else if (M.model == 3) cm_perfect(M, er, s, p, epb, dt);
+        else if (M.model == 4) cm_drucker_prager(M, er, s, p, epb, dt);
+        if (M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
+      }
+      // periodic wrap with period L-2h (TimeIntegration.cpp:80-108)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (q[a] > I.wrapq[a]) q[a] = I.oneq + q[a] % I.wrapq[a];
+        if (q[a] < I.oneq) q[a] += I.wrapq[a] - I.oneq;
+      }
+      if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+
+      // ---- store the new state at its sorted slot ----
+#pragma unroll
+      for (int a = 0; a < 3; a++) B[(F_QX + a) * cap + out] = (uint32_t)q[a];
+#pragma unroll
+      for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
+#pragma unroll
+      for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
+#pragma unroll
+      for (int a = 0; a < 6; a++) B[(F_E0 + a) * cap + out] = u32(e[a]);
+      if (plastic) {
+#pragma unroll
+        for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = u32(p[a]);
+        B[F_EPB * cap + out] = u32(epb);
+      }
+      if (M.carry_damage) {
+        B[F_DMG * cap + out] = u32(dmg);
+        B[F_SMAX * cap + out] = u32(smax);
+      }
+      B[F_RHO * cap + out] = u32(rho);
+      B[F_MASS * cap + out] = u32(mass);
+      B[F_TAG * cap + out] = tag;
+      B[F_PID * cap + out] = (uint32_t)pid;
+
+      // ---- new binning key + histogram ----
+      bad = false;
+#pragma unroll
+      for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+      const int knew = key_of(I, b);
+      keys_out[out] = knew;
+      d.rank[out] = atomicAdd(&d.count[knew], 1);
+      if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+    }
+
+    // ---- P2G of the (new) state (Extrapolation.cpp:166-286) ----
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+    P2GTerms T;
+    const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
+    T.m = mp;
+#pragma unroll
+    for (int a = 0; a < 3; a++) T.mv[a] = mp * v[a];
+    const float nV = -(mp / rho);
+#pragma unroll
+    for (int a = 0; a < 6; a++) T.vs[a] = nV * s[a];
+    const int r0 = b[0] - o[0] + 1, r1 = b[1] - o[1] + 1, r2 = b[2] - o[2] + 1;
+    const bool inside = (r0 >= 0 && r1 >= 0 && r2 >= 0 && r0 + 2 < P2G_R && r1 + 2 < P2G_R && r2 + 2 < P2G_R);
+    if (inside) {
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++) {
+          const float Aw = W[1].w[jj] * W[2].w[kk];
+          const float By = W[1].g[jj] * W[2].w[kk];
+          const float Bz = W[1].w[jj] * W[2].g[kk];
+          const int row = r0 + P2G_R * ((r1 + jj) + P2G_R * (r2 + kk));
+#pragma unroll
+          for (int ii2 = 0; ii2 < 3; ii2++) {
+            float tt[7];
+            node_terms(T, W[0].w[ii2] * Aw, W[0].g[ii2] * Aw, W[0].w[ii2] * By, W[0].w[ii2] * Bz, tt);
+#pragma unroll
+            for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], tt[f]);
+          }
+        }
+    } else {
+      p2g_global(d, I, b, W, T);
+    }
+  }
+  __syncthreads();
+
+  // ---- flush the LDS tile: one set of global atomics per touched node ----
+  for (int n = tid; n < NODES_P2G; n += WG) {
+    const float m = s_acc[0][n];
+    if (m == 0.0f) continue;
+    const int g[3] = {o[0] - 1 + n % P2G_R, o[1] - 1 + (n / P2G_R) % P2G_R, o[2] - 1 + n / (P2G_R * P2G_R)};
+    if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
+    int c[3];
+    const long long ci = canon_index(I, g, c);
+    float *dst = d.acc + ci * NODE_W;
+#pragma unroll
+    for (int f = 0; f < 7; f++) unsafeAtomicAdd(dst + f, s_acc[f][n]);
+    flag_brick(d, I, c);
+  }
+}
+
+// FP64 audit: per block sums of mass, momentum, kinetic energy, a stress-energy proxy and
+// non-finite count.
+constexpr int AUD = 8;
+__global__ __launch_bounds__(WG) void k_audit(DevPtrs d) {
+  __shared__ double sh[AUD][WG];
+  const int s = blockIdx.x * WG + threadIdx.x;
+  double r[AUD] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < d.np) {
+    const uint32_t *A = d.pbuf[d.ctrl->parity];
+    const long long cap = d.cap;
+    const double m = f32(A[F_MASS * cap + s]);
+    double v[3], ss = 0;
+    bool fin = true;
+    for (int a = 0; a < 3; a++) {
+      const float x = f32(A[(F_VX + a) * cap + s]);
+      fin = fin && isfinite(x);
+      v[a] = x;
+    }
+    for (int a = 0; a < 6; a++) {
+      const float x = f32(A[(F_S0 + a) * cap + s]);
+      fin = fin && isfinite(x);
+      ss += (a < 3 ? 1.0 : 2.0) * (double)x * (double)x;
+    }
+    const double rho = f32(A[F_RHO * cap + s]);
+    r[0] = m;
+    r[1] = m * v[0]; r[2] = m * v[1]; r[3] = m * v[2];
+    r[4] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    r[5] = 0.5 * (m / rho) * ss;
+    r[6] = fin ? 0.0 : 1.0;
+  }
+  for (int k = 0; k < AUD; k++) sh[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < AUD; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < AUD) d.audit[(long long)blockIdx.x * AUD + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(WG) void k_audit_final(DevPtrs d, int nblocks) {
+  __shared__ double sh[AUD][WG];
+  double r[AUD] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += WG)
+    for (int k = 0; k < AUD; k++) r[k] += d.audit[(long long)b * AUD + k];
+  for (int k = 0; k < AUD; k++) sh[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < AUD; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < AUD) d.audit[(long long)nblocks * AUD + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int blocks(long long n, int per) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+int scan_items_per_block() { return SCAN_BLOCK; }
+
+void launch_set_ctrl(const DevPtrs &d, int step, double t, int reset, void *st) {
+  hipLaunchKernelGGL(k_set_ctrl, dim3(blocks(d.ninst, WG)), dim3(WG), 0, S(st), d, step, t, reset);
+}
+void launch_keys(const DevPtrs &d, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_keys, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
+}
+void launch_hist(const DevPtrs &d, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
+}
+void launch_scan_partial(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_scan_partial, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d);
+}
+void launch_scan_final(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_scan_final, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d);
+}
+void launch_place(const DevPtrs &d, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_place, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
+}
+void launch_p2g_only(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_g2p2g<false>, dim3(d.ntiles), dim3(WG), 0, S(st), d);
+}
+void launch_grid(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+}
+void launch_g2p2g(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_g2p2g<true>, dim3(d.ntiles), dim3(WG), 0, S(st), d);
+}
+void launch_advance(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_advance, dim3(blocks(d.ninst, WG)), dim3(WG), 0, S(st), d);
+}
+void launch_audit(const DevPtrs &d, void *st) {
+  const int nb = blocks(d.np, WG);
+  hipLaunchKernelGGL(k_audit, dim3(nb), dim3(WG), 0, S(st), d);
+  hipLaunchKernelGGL(k_audit_final, dim3(1), dim3(WG), 0, S(st), d, nb);
+}
+void launch_stable_order(const DevPtrs &d, int *order, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_stable_order, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d, order);
+}
+
+}  // namespace gy

@@ -1,0 +1,189 @@
+"""ctypes binding of libgraphyt_amd.so and a thin Python context over the C ABI.
+
+Loading fails loudly (ImportError) when the library has not been built; there is no
+CPU fallback anywhere in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (GY_OK, GyAudit, GyGrid, GyMaterial, GyParams, GyParticles, ParticleArrays,
+                  material_array)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libgraphyt_amd.so")
+
+# Every entry point declared in include/graphyt_amd.h: name -> (restype, argtypes).
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "gy_create": (C.c_int, [_PP, C.POINTER(GyGrid), C.POINTER(GyMaterial), C.c_int32, C.POINTER(GyParams)]),
+    "gy_set_node_bc": (C.c_int, [_P, C.c_int32, _I32P, _DP]),
+    "gy_upload": (C.c_int, [_P, C.POINTER(GyParticles)]),
+    "gy_set_dt": (C.c_int, [_P, C.c_double]),
+    "gy_get_dt": (C.c_double, [_P]),
+    "gy_iterate": (C.c_int, [_P, C.c_double, C.c_int32]),
+    "gy_iterate_n": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32]),
+    "gy_synchronize": (C.c_int, [_P]),
+    "gy_download": (C.c_int, [_P, C.POINTER(GyParticles)]),
+    "gy_audit": (C.c_int, [_P, C.POINTER(GyAudit)]),
+    "gy_debug_binning": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I64P, _I64P]),
+    "gy_counting_sort": (C.c_int, [_I32P, C.c_int64, C.c_int64, _I32P, _I32P, _I32P]),
+    "gy_set_option": (C.c_int, [_P, C.c_int32, C.c_int64]),
+    "gy_kernel_count": (C.c_int, []),
+    "gy_kernel_name": (C.c_char_p, [C.c_int32]),
+    "gy_kernel_stats": (C.c_int, [_P, C.c_int32, _I64P, _DP]),
+    "gy_reset_kernel_stats": (C.c_int, [_P]),
+    "gy_stream": (C.c_void_p, [_P]),
+    "gy_last_error": (C.c_char_p, [_P]),
+    "gy_version": (C.c_char_p, []),
+    "gy_destroy": (None, [_P]),
+}
+
+OPT_USE_GRAPH, OPT_PROFILE, OPT_TILE = 1, 2, 3
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m graphyt_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"graphyt_amd error {code}: {msg}")
+        self.code = code
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+class Context:
+    """One device context (gy_ctx) holding a Scene's grid, materials and particles."""
+
+    def __init__(self, scene, graph: bool = True):
+        self.lib = load()
+        self.scene = scene
+        self._mats = material_array(scene.mats)
+        self._grid = scene.grid()
+        self._params = scene.params()
+        h = C.c_void_p()
+        rc = self.lib.gy_create(C.byref(h), C.byref(self._grid), self._mats, len(scene.mats), C.byref(self._params))
+        self.h = h
+        self._check(rc)
+        if scene.node_bc_type is not None:
+            t = _i32(scene.node_bc_type)
+            v = np.ascontiguousarray(scene.node_bc_val, np.float64)
+            self._check(self.lib.gy_set_node_bc(self.h, len(t), t.ctypes.data_as(_I32P), v.ctypes.data_as(_DP)))
+        self._check(self.lib.gy_upload(self.h, C.byref(scene.particles.view())))
+        self.set_option(OPT_USE_GRAPH, int(graph))
+
+    def _check(self, rc):
+        if rc != GY_OK:
+            msg = self.lib.gy_last_error(self.h).decode() if self.h else ""
+            raise GyError(rc, msg)
+
+    @property
+    def dt(self) -> float:
+        return self.lib.gy_get_dt(self.h)
+
+    def set_dt(self, dt: float):
+        self._check(self.lib.gy_set_dt(self.h, dt))
+
+    def set_option(self, opt: int, value: int):
+        self._check(self.lib.gy_set_option(self.h, opt, value))
+
+    def iterate(self, t: float, step: int):
+        self._check(self.lib.gy_iterate(self.h, t, step))
+
+    def run(self, nsteps: int, step0: int = 0, t0: float | None = None):
+        if t0 is None:
+            t0 = step0 * self.dt
+        self._check(self.lib.gy_iterate_n(self.h, t0, step0, nsteps))
+
+    def synchronize(self):
+        self._check(self.lib.gy_synchronize(self.h))
+
+    def download(self, fields=("pos", "vel", "stress", "strain", "plastic_strain", "acc_plastic_strain",
+                               "density", "mass", "damage", "smax", "volume", "displacement", "cell_id")):
+        pa = ParticleArrays.empty(self.scene.n, fields)
+        self._check(self.lib.gy_download(self.h, C.byref(pa.view())))
+        return pa.arrays
+
+    def audit(self) -> dict:
+        a = GyAudit()
+        self._check(self.lib.gy_audit(self.h, C.byref(a)))
+        return {"mass": a.mass, "momentum": tuple(a.momentum), "kinetic_energy": a.kinetic_energy,
+                "strain_energy_proxy": a.strain_energy_proxy, "nonfinite": a.nonfinite,
+                "out_of_range": a.out_of_range}
+
+    def binning(self) -> dict:
+        nk, ns = C.c_int64(), C.c_int64()
+        self._check(self.lib.gy_debug_binning(self.h, None, None, None, None, C.byref(nk), C.byref(ns)))
+        keys = np.zeros(ns.value, np.int32)
+        counts = np.zeros(nk.value, np.int32)
+        offsets = np.zeros(nk.value, np.int32)
+        order = np.zeros(ns.value, np.int32)
+        self._check(self.lib.gy_debug_binning(self.h, keys.ctypes.data_as(_I32P), counts.ctypes.data_as(_I32P),
+                                              offsets.ctypes.data_as(_I32P), order.ctypes.data_as(_I32P),
+                                              C.byref(nk), C.byref(ns)))
+        return {"keys": keys, "counts": counts, "offsets": offsets, "order": order}
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for i in range(self.lib.gy_kernel_count()):
+            n, ms = C.c_int64(), C.c_double()
+            self._check(self.lib.gy_kernel_stats(self.h, i, C.byref(n), C.byref(ms)))
+            if n.value:
+                out[self.lib.gy_kernel_name(i).decode()] = (n.value, ms.value)
+        return out
+
+    def reset_kernel_stats(self):
+        self._check(self.lib.gy_reset_kernel_stats(self.h))
+
+    @property
+    def stream(self) -> int:
+        return self.lib.gy_stream(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.gy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def counting_sort(keys, nkeys: int):
+    """Stable counting sort of int32 keys in [0, nkeys) on the GPU (gy_counting_sort)."""
+    lib = load()
+    keys = _i32(keys)
+    counts = np.zeros(nkeys, np.int32)
+    offsets = np.zeros(nkeys, np.int32)
+    order = np.zeros(len(keys), np.int32)
+    rc = lib.gy_counting_sort(keys.ctypes.data_as(_I32P), len(keys), nkeys, counts.ctypes.data_as(_I32P),
+                              offsets.ctypes.data_as(_I32P), order.ctypes.data_as(_I32P))
+    if rc != GY_OK:
+        raise GyError(rc, "gy_counting_sort failed")
+    return counts, offsets, order
