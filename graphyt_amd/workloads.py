@@ -168,6 +168,14 @@ def dp_column_small(h=0.02, steps=50, **kw) -> Scene:
                      steps=steps, **kw)
 
 
+def dp_column_mid(steps=1000, **kw) -> Scene:
+    """The Drucker-Prager column at the benchmark resolution (h = 0.01, psep = 0.005, same
+    material, BCs and dt) on a 0.6x0.5x0.3 m domain: 20x40x20 = 16,000 particles, small
+    enough for the FP64 oracle to run 1000 steps in seconds."""
+    return dp_column(h=0.01, L=(0.6, 0.5, 0.3), lo=(0.03, 0.03, 0.1), size=(0.1, 0.2, 0.1),
+                     steps=steps, **kw)
+
+
 def impact_block(h=0.01, n_nodes=24, count=20000, K=1e11, G=4e10, sy=5e8, v=100.0,
                  model=3, seed=SEED) -> Scene:
     """Perfect-plastic steel block moving at -v along x toward a fixed grid_vx wall."""
