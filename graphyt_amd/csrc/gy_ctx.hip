@@ -273,6 +273,16 @@ int alloc_grid(gy_ctx *c) {
   if ((rc = dalloc(c, &d.count, (size_t)d.nkeys))) return rc;
   if ((rc = dalloc(c, &d.offsets, (size_t)d.nkeys + 1))) return rc;
   if ((rc = dalloc(c, &d.blocksum, (size_t)d.nblocks_scan))) return rc;
+  if ((rc = dalloc(c, &d.blocktiles, (size_t)d.nblocks_scan))) return rc;
+  if ((rc = dalloc(c, &d.active, (size_t)tile_base + 1))) return rc;
+  if ((rc = dalloc(c, &d.tile_epoch, (size_t)tile_base))) return rc;
+  if ((rc = dalloc(c, &d.partial, (size_t)tile_base * PARTIAL_W))) return rc;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+    d.persist_wgs = 8 * ((cus * 3 + 7) / 8);  // three resident 256-thread workgroups per CU
+  }
+  d.cell_lanes = 1;
   if ((rc = dalloc(c, &d.inst, c->inst.size()))) return rc;
   if ((rc = dalloc(c, &d.mats, c->mats.size()))) return rc;
   if ((rc = dalloc(c, &d.ctrl, 1))) return rc;
@@ -471,6 +481,21 @@ int gy_upload(gy_ctx *c, const gy_particles *in) {
     H.carry_plastic[m] = (uint8_t)M.carry_plastic;
     H.carry_damage[m] = (uint8_t)M.carry_damage;
   }
+  // model-set bits for the fused kernel (elastic 1, Newtonian 2, perfect 4, DP 8, GK 16,
+  // particle BCs 32); see k_g2p2g in gy_kernels.hip
+  int kmask = 0;
+  std::vector<uint8_t> used(nmat, 0);
+  for (int i = 0; i < n; i++) used[in->material ? in->material[i] : 0] = 1;
+  for (int m = 0; m < nmat; m++) {
+    if (!used[m]) continue;
+    const gy_material &gm = H.mats[m];
+    kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : 8;
+    if (gm.is_damage && gm.damage_model == 1) kmask |= 16;
+    if (c->mats[I.mat_base + m].carry_damage && !(gm.is_damage && gm.damage_model == 1)) kmask |= 16;
+  }
+  for (int i = 0; in->bc_type && i < n; i++)
+    if (in->bc_type[i]) { kmask |= 32; break; }
+  d.kmask = kmask;
   H.nparticles = n;
   I.pstart = 0;
   I.pcount = n;
@@ -664,7 +689,8 @@ int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *cou
   if ((rc = dalloc(&tmp, &d.key[0], (size_t)n)) || (rc = dalloc(&tmp, &d.rank, (size_t)n)) ||
       (rc = dalloc(&tmp, &d.tmp, (size_t)n)) || (rc = dalloc(&tmp, &d.count, (size_t)nkeys)) ||
       (rc = dalloc(&tmp, &d.offsets, (size_t)nkeys + 1)) || (rc = dalloc(&tmp, &d.blocksum, (size_t)d.nblocks_scan)) ||
-      (rc = dalloc(&tmp, &d.ctrl, 1))) {
+      (rc = dalloc(&tmp, &d.blocktiles, (size_t)d.nblocks_scan)) ||
+      (rc = dalloc(&tmp, &d.active, (size_t)(nkeys + TILE3 - 1) / TILE3 + 1)) || (rc = dalloc(&tmp, &d.ctrl, 1))) {
     rc = GY_ERR_HIP;
   } else {
     int *dord = nullptr;
@@ -691,6 +717,7 @@ int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *cou
     dfree(dord);
   }
   dfree(d.key[0]); dfree(d.rank); dfree(d.tmp); dfree(d.count); dfree(d.offsets); dfree(d.blocksum); dfree(d.ctrl);
+  dfree(d.blocktiles); dfree(d.active);
   (void)hipStreamDestroy(tmp.stream);
   tmp.stream = nullptr;
   return rc;
@@ -702,6 +729,12 @@ int gy_set_option(gy_ctx *c, int32_t option, int64_t value) {
     case GY_OPT_USE_GRAPH: c->use_graph = value != 0; if (!c->use_graph) drop_graph(c); return GY_OK;
     case GY_OPT_PROFILE: c->profile = value != 0; return GY_OK;
     case GY_OPT_TILE: return value == TILE ? GY_OK : fail(c, GY_ERR_UNSUPPORTED, "tile edge %lld not built", (long long)value);
+    case GY_OPT_CELL_LANES: c->d.cell_lanes = value != 0; drop_graph(c); return GY_OK;
+    case GY_OPT_PERSIST_WGS:
+      if (value < 8 || value > 65536) return fail(c, GY_ERR_INVALID, "persistent workgroups must be in [8, 65536]");
+      c->d.persist_wgs = (int)(8 * ((value + 7) / 8));
+      drop_graph(c);
+      return GY_OK;
     default: return fail(c, GY_ERR_INVALID, "unknown option %d", option);
   }
 }
@@ -737,6 +770,7 @@ void gy_destroy(gy_ctx *c) {
   dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.bc_type); dfree(d.bc_val);
   dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
   dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
+  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.partial);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -25,6 +25,8 @@ constexpr int BRICK = 4;                   // canonical nodes per grid-brick edg
 constexpr int G2P_R = TILE + 2;            // staged VEL region edge (nodes)
 constexpr int P2G_R = TILE + 4;            // LDS accumulator region edge (nodes), halo -1/+1
 constexpr int NODE_W = 8;                  // floats per node record
+constexpr int P2G_NODES = P2G_R * P2G_R * P2G_R;  // nodes of a tile's P2G region
+constexpr int PARTIAL_W = 7 * P2G_NODES;   // floats of one tile's partial sums
 
 enum PField : int {
   F_QX = 0, F_QY, F_QZ,                     // fixed-point position (int32 bits)
@@ -101,10 +103,13 @@ struct MatDev {
 // Step control read by every kernel of a step (updated on the device, so a captured
 // graph can be replayed for many steps).
 struct Ctrl {
-  int step;      // reference step number (half kick at step == 1, Grid.cpp:126)
-  int parity;    // which state buffer is the source this step
-  int status;    // sticky error bits (1 = particle stencil left the grid)
-  int oor;       // count of out-of-range particles (clamped)
+  int step;          // reference step number (half kick at step == 1, Grid.cpp:126)
+  int parity;        // which state buffer is the source this step
+  int status;        // sticky error bits (1 = particle stencil left the grid)
+  int oor;           // count of out-of-range particles (clamped)
+  int epoch;         // tile partials written with tile_epoch == epoch are this step's P2G
+  int active_tiles;  // tiles holding particles in the current binning
+  int next[8];       // per-XCD work counters of the persistent particle kernel
 };
 
 // All device pointers of a context (passed by value to kernels).
@@ -115,8 +120,12 @@ struct DevPtrs {
   int *tmp;              // key-grouped slot list of the current binning
   int *count;            // per-key histogram (nkeys)
   int *offsets;          // per-key exclusive offsets (nkeys + 1)
-  int *blocksum;         // scan partials
-  float *acc;            // NODE_W per global node
+  int *blocksum;         // scan partials (particle counts per scan block)
+  int *blocktiles;       // scan partials (active tiles per scan block)
+  int *active;           // active tile list, ascending tile id
+  int *tile_epoch;       // per tile: epoch of its last written partials
+  float *partial;        // per tile: P2G sums of its LDS region, [7][P2G_R^3] floats
+  float *acc;            // NODE_W per global node: P2G of particles outside their tile's region
   float *vel;            // NODE_W per global node
   uint8_t *brick_flag;   // per global canonical brick
   uint8_t *bc_type;      // per global node
@@ -137,6 +146,9 @@ struct DevPtrs {
   int ntiles;
   int nbricks;
   int nblocks_scan;
+  int kmask;             // model-set bits of the fused kernel specialization
+  int persist_wgs;       // workgroups of the persistent particle kernel (multiple of 8)
+  int cell_lanes;        // 1: lane = cell of the tile; 0: lane = consecutive particle
 };
 
 // Kernel identifiers for per-kernel timing.

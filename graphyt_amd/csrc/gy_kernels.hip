@@ -28,7 +28,7 @@ constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
 constexpr int MAXMAT = 16;
 constexpr int NODES_G2P = G2P_R * G2P_R * G2P_R;
-constexpr int NODES_P2G = P2G_R * P2G_R * P2G_R;
+constexpr int NODES_P2G = P2G_NODES;
 
 __device__ __forceinline__ float f32(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
@@ -61,6 +61,8 @@ __device__ __forceinline__ int key_of(const InstDesc &I, const int b[3]) {
 struct Axis {
   float w[3], g[3];
 };
+// one of three values by a loop index k in {0,1,2}, by value so no array is spilled
+__device__ __forceinline__ float sel3(int k, float a0, float a1, float a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
 __device__ __forceinline__ Axis gimp3(float u, float lp, float inv_lp, float inv_h) {
   Axis A;
   const float d0 = -u, d1 = 1.0f - u, d2 = 2.0f - u;
@@ -342,6 +344,7 @@ __global__ void k_set_ctrl(DevPtrs d, int step, double t, int reset) {
   if (i == 0) {
     d.ctrl->step = step;
     if (reset) { d.ctrl->parity = 0; d.ctrl->status = 0; d.ctrl->oor = 0; }
+    if (d.ctrl->epoch == 0) d.ctrl->epoch = 1;  // tile_epoch starts at 0: never current
   }
 }
 
@@ -354,6 +357,7 @@ __global__ void k_advance(DevPtrs d) {
   if (i == 0) {
     d.ctrl->step += 1;
     d.ctrl->parity ^= 1;
+    d.ctrl->epoch += 1;
   }
 }
 
@@ -371,14 +375,14 @@ __global__ __launch_bounds__(WG) void k_keys(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
   if (s >= d.np) return;
   const int par = d.ctrl->parity;
-  const uint32_t *A = d.pbuf[par];
+  const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   const InstDesc &I = d.inst[inst_of_slot(d, s)];
   bool bad = false;
   int b[3];
   for (int a = 0; a < 3; a++)
     b[a] = stencil_base((int)A[(F_QX + a) * d.cap + s], I.lpq, I.fbits, I.n[a] - 3, bad);
   const int k = key_of(I, b);
-  d.key[par][s] = k;
+  (par ? d.key[1] : d.key[0])[s] = k;
   d.rank[s] = atomicAdd(&d.count[k], 1);
   if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
 }
@@ -419,6 +423,10 @@ __global__ __launch_bounds__(WG) void k_hist(DevPtrs d) {
   d.rank[s] = atomicAdd(&d.count[d.key[0][s]], 1);
 }
 
+// Scan blocks cover SCAN_BLOCK keys = SCAN_BLOCK / TILE3 whole tiles; four consecutive
+// threads (16 keys each) hold one tile, so tile occupancy is a 4-lane shuffle sum.
+static_assert(TILE3 == 4 * SCAN_ITEMS, "a tile is four scan threads");
+
 __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d) {
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
@@ -427,15 +435,26 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d) {
     const long long k = base + j;
     if (k < d.nkeys) s += d.count[k];
   }
+  int ts = s + __shfl_xor(s, 1, 64);
+  ts += __shfl_xor(ts, 2, 64);
+  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
   const int tot = block_sum(s);
-  if (threadIdx.x == 0) d.blocksum[blockIdx.x] = tot;
+  const int tact = block_sum(act);
+  if (threadIdx.x == 0) {
+    d.blocksum[blockIdx.x] = tot;
+    d.blocktiles[blockIdx.x] = tact;
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
-  // prefix of all earlier blocks
-  int pre = 0;
-  for (int j = threadIdx.x; j < (int)blockIdx.x; j += WG) pre += d.blocksum[j];
+  // prefixes of all earlier blocks (particles and active tiles)
+  int pre = 0, pret = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += WG) {
+    pre += d.blocksum[j];
+    pret += d.blocktiles[j];
+  }
   pre = block_sum(pre);
+  pret = block_sum(pret);
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int c[SCAN_ITEMS];
   int s = 0;
@@ -456,13 +475,24 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     }
     run += c[j];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) d.offsets[d.nkeys] = pre + tot;
+  // compacted active tile list in ascending tile order
+  int ts = s + __shfl_xor(s, 1, 64);
+  ts += __shfl_xor(ts, 2, 64);
+  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
+  int ttot;
+  const int pos = pret + block_exclusive_scan(act, ttot);
+  if (act) d.active[pos] = (int)(base / TILE3);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    d.offsets[d.nkeys] = pre + tot;
+    d.ctrl->active_tiles = pret + ttot;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(WG) void k_place(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
   if (s >= d.np) return;
-  const int k = d.key[d.ctrl->parity][s];
+  const int k = (d.ctrl->parity ? d.key[1] : d.key[0])[s];
   d.tmp[d.offsets[k] + d.rank[s]] = s;
 }
 
@@ -481,12 +511,44 @@ __global__ __launch_bounds__(WG) void k_stable_order(DevPtrs d, int *order) {
   const int i = blockIdx.x * WG + threadIdx.x;
   if (i >= d.np) return;
   const int slot = d.tmp[i];
-  order[stable_pos(d, d.key[d.ctrl->parity], i, slot)] = slot;
+  order[stable_pos(d, (d.ctrl->parity ? d.key[1] : d.key[0]), i, slot)] = slot;
 }
 
 // ---------------------------------------------------------------------------------------
-// Grid update on the active canonical bricks (one wave per 4x4x4 brick).
+// Sum of the tile partials covering actual node (a0, a1, a2): a node n lies in the P2G
+// region [o-1, o+TILE+3) of the tiles with o = TILE*(tt-1) and TILE*tt, tt = (n+1)/TILE.
+// Only tiles written this step (tile_epoch == epoch) count. Fixed order: deterministic.
+__device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc &I, int epoch, int a0, int a1,
+                                                int a2, float acc7[7]) {
+  const int t0 = (a0 + 1) / TILE, t1 = (a1 + 1) / TILE, t2 = (a2 + 1) / TILE;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++) {
+    const int tz = t2 - 1 + dz;
+    if (tz < 0 || tz >= I.nt[2]) continue;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+      const int ty = t1 - 1 + dy;
+      if (ty < 0 || ty >= I.nt[1]) continue;
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        const int tx = t0 - 1 + dx;
+        if (tx < 0 || tx >= I.nt[0]) continue;
+        const int t = I.tile_base + tx + I.nt[0] * (ty + I.nt[1] * tz);
+        if (d.tile_epoch[t] != epoch) continue;
+        const int l = (a0 - TILE * tx + 1) + P2G_R * ((a1 - TILE * ty + 1) + P2G_R * (a2 - TILE * tz + 1));
+        const float *pp = d.partial + (long long)t * PARTIAL_W + l;
+#pragma unroll
+        for (int f = 0; f < 7; f++) acc7[f] += pp[f * P2G_NODES];
+      }
+    }
+  }
+}
+
+// Grid update on the active canonical bricks (one wave per 4x4x4 brick): gather the P2G
+// partials of every actual node of the periodic class, then solve.
 __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
+  // the particle kernel that follows walks the active tile list from the start
+  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gb = blockIdx.x * (WG / 64) + wave;
   if (gb >= d.nbricks) return;
@@ -495,29 +557,47 @@ __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
   const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
-  const int step = d.ctrl->step;
+  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
   if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
+    // P2G sums over the periodic class (Grid.cpp:320-602: node i and node N-3+i share sums)
+    float a7[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int mz = 0; mz < 2; mz++) {
+      if (mz && c2 >= 3) break;
+#pragma unroll
+      for (int my = 0; my < 2; my++) {
+        if (my && c1 >= 3) break;
+#pragma unroll
+        for (int mx = 0; mx < 2; mx++) {
+          if (mx && c0 >= 3) break;
+          gather_partials(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], c2 + mz * I.cper[2], a7);
+        }
+      }
+    }
+    // particles scattered straight to the grid (outside their tile's region)
     const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
     float4 *ap = reinterpret_cast<float4 *>(d.acc + ci * NODE_W);
-    const float4 a0 = ap[0], a1 = ap[1];
-    ap[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ap[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float m = a0.x;
+    const float4 x0 = ap[0], x1 = ap[1];
+    if (x0.x != 0.f || x0.y != 0.f || x0.z != 0.f || x0.w != 0.f || x1.x != 0.f || x1.y != 0.f || x1.z != 0.f) {
+      a7[0] += x0.x; a7[1] += x0.y; a7[2] += x0.z; a7[3] += x0.w;
+      a7[4] += x1.x; a7[5] += x1.y; a7[6] += x1.z;
+      ap[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+      ap[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const float m = a7[0];
     float v[3] = {0.f, 0.f, 0.f}, acc[3] = {0.f, 0.f, 0.f};
     if (m != 0.0f) {
       // Grid.cpp:14-16, 220-243: v = p/m; a = f/m (gravity enters as m*g, the sum of the
       // per-pair N*m_p*g terms of Extrapolation.cpp:226-228); v += a*dt (0.5*dt at step 1)
       const float kick = (step == 1) ? 0.5f * I.dt : I.dt;
-      const float p[3] = {a0.y, a0.z, a0.w};
-      const float f[3] = {a1.x + m * I.gravity[0], a1.y + m * I.gravity[1], a1.z + m * I.gravity[2]};
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        acc[k] = f[k] / m;
-        v[k] = p[k] / m + acc[k] * kick;
+        acc[k] = (a7[4 + k] + m * I.gravity[k]) / m;
+        v[k] = a7[1 + k] / m + acc[k] * kick;
       }
     }
-    // fan out to the actual nodes of the periodic class (Grid.cpp:320-602) with each
-    // node's own BC (Grid.cpp:862-1031)
+    // fan out to the actual nodes of the periodic class with each node's own BC
+    // (Grid.cpp:862-1031)
 #pragma unroll
     for (int mz = 0; mz < 2; mz++) {
       if (mz && c2 >= 3) break;
@@ -578,283 +658,357 @@ __device__ __forceinline__ long long canon_index(const InstDesc &I, const int g[
 }
 
 // P2G of one particle straight into the global accumulators (used when its stencil lies
-// outside the tile's LDS region, e.g. after a periodic wrap).
-__device__ void p2g_global(const DevPtrs &d, const InstDesc &I, const int b[3], const Axis W[3],
-                           const P2GTerms &T) {
-#pragma unroll 1
+// outside the tile's LDS region, e.g. after a periodic wrap). Kept out of line and fully
+// unrolled so the hot path never materialises the weight arrays in scratch.
+__device__ __noinline__ void p2g_global(float *acc, uint8_t *brick_flag, const InstDesc *Ip, int b0, int b1, int b2,
+                                        Axis W0, Axis W1, Axis W2, P2GTerms T) {
+  const InstDesc &I = *Ip;
+  const int b[3] = {b0, b1, b2};
+#pragma unroll
   for (int k = 0; k < 3; k++)
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < 3; j++)
-#pragma unroll 1
+#pragma unroll
       for (int i = 0; i < 3; i++) {
-        const float A = W[1].w[j] * W[2].w[k];
-        const float w = W[0].w[i] * A;
+        const float A = W1.w[j] * W2.w[k];
+        const float w = W0.w[i] * A;
         if (w == 0.0f) continue;
         float t[7];
-        node_terms(T, w, W[0].g[i] * A, W[0].w[i] * W[1].g[j] * W[2].w[k],
-                   W[0].w[i] * W[1].w[j] * W[2].g[k], t);
+        node_terms(T, w, W0.g[i] * A, W0.w[i] * W1.g[j] * W2.w[k], W0.w[i] * W1.w[j] * W2.g[k], t);
         const int g[3] = {b[0] + i, b[1] + j, b[2] + k};
         int c[3];
         const long long ci = canon_index(I, g, c);
-        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&d.acc[ci * NODE_W + f], t[f]);
-        flag_brick(d, I, c);
+#pragma unroll
+        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&acc[ci * NODE_W + f], t[f]);
+        brick_flag[I.brick_base + c[0] / BRICK + I.nb[0] * (c[1] / BRICK + I.nb[1] * (c[2] / BRICK))] = 1;
       }
 }
 
-// The fused particle kernel. UPDATE = false: P2G of the current state only (first step
-// after an upload). UPDATE = true: G2P + update + constitutive + P2G of the new state.
-template <bool UPDATE>
-__global__ __launch_bounds__(WG) void k_g2p2g(DevPtrs d) {
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int k0 = t * TILE3;
-  const int p0 = d.offsets[k0], p1 = d.offsets[k0 + TILE3];
-  if (p0 == p1) return;
+// Model-set bits: the fused kernel is instantiated per set of constitutive models a
+// context uses, so a single-material run carries only its own model's registers.
+constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32;
+constexpr int KM_ALL = 63;
 
-  __shared__ float s_vel[6][NODES_G2P];
-  __shared__ float s_acc[7][NODES_P2G];
-  __shared__ MatDev s_mat[MAXMAT];
-
-  const int ii = d.tile_inst[t];
-  const InstDesc I = d.inst[ii];
-  const int lt = t - I.tile_base;
-  const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
-  const int par = d.ctrl->parity;
-  const uint32_t *A = d.pbuf[par];
-  uint32_t *B = d.pbuf[par ^ 1];
+// One particle of the fused kernel: G2P + update + constitutive + store (UPDATE), then P2G
+// of the (new) state into the tile's FP64 LDS accumulators. `i` is the particle's index in
+// the key-grouped slot list and [seg_lo, seg_hi) its cell's segment there.
+template <bool UPDATE, int MASK>
+__device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
+                                              double tl, int i, int seg_lo, int seg_hi,
+                                              float (*s_vel)[NODES_G2P], double (*s_acc)[NODES_P2G],
+                                              const MatDev *s_mat, int par) {
+  const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
+  uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
   const long long cap = d.cap;
-  const int tid = threadIdx.x;
+  const int slot = d.tmp[i];
+  int q[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) q[a] = (int)A[(F_QX + a) * cap + slot];
+  const uint32_t tag = A[F_TAG * cap + slot];
+  const MatDev &M = s_mat[tag_mat(tag)];
+  float v[3], s[6];
+#pragma unroll
+  for (int a = 0; a < 3; a++) v[a] = f32(A[(F_VX + a) * cap + slot]);
+#pragma unroll
+  for (int a = 0; a < 6; a++) s[a] = f32(A[(F_S0 + a) * cap + slot]);
+  float rho = f32(A[F_RHO * cap + slot]);
+  const float mass = f32(A[F_MASS * cap + slot]);
+  bool bad = false;
+  int b[3];
+  Axis W[3];
 
-  for (int n = tid; n < 7 * NODES_P2G; n += WG) (&s_acc[0][0])[n] = 0.0f;
-  if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
   if (UPDATE) {
-    for (int n = tid; n < NODES_G2P; n += WG) {
-      const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
-      const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
-      float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-      if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
-        const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
-        const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
-        v0 = vp[0];
-        v1 = vp[1];
-      }
-      s_vel[0][n] = v0.x; s_vel[1][n] = v0.y; s_vel[2][n] = v0.z;
-      s_vel[3][n] = v0.w; s_vel[4][n] = v1.x; s_vel[5][n] = v1.y;
+    // stable counting-sort slot: ascending slot order inside the cell
+    int out = seg_lo;
+    for (int j = seg_lo; j < seg_hi; j++) out += (d.tmp[j] < slot) ? 1 : 0;
+    // ---- G2P (Extrapolation.cpp:127-163, 288-367) ----
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+      W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
     }
-  }
-  __syncthreads();
-
-  const double tl = UPDATE ? d.t_last[ii] : 0.0;
-  const int *keys_in = d.key[par];
-  int *keys_out = d.key[par ^ 1];
-
-  for (int i = p0 + tid; i < p1; i += WG) {
-    const int slot = d.tmp[i];
-    int q[3];
+    float vg[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
+    float Lx[3] = {0, 0, 0}, Ly[3] = {0, 0, 0}, Lz[3] = {0, 0, 0};  // d v_b / d x_a
+    int r0 = b[0] - o[0], r1 = b[1] - o[1], r2 = b[2] - o[2];
+    if ((unsigned)r0 >= (unsigned)TILE || (unsigned)r1 >= (unsigned)TILE || (unsigned)r2 >= (unsigned)TILE) {
+      // a particle listed under a tile it is not in: never expected; keep LDS reads in range
+      atomicOr(&d.ctrl->status, 2);
+      r0 = min(max(r0, 0), TILE - 1);
+      r1 = min(max(r1, 0), TILE - 1);
+      r2 = min(max(r2, 0), TILE - 1);
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < 3; kk++) {
+      const float wz = sel3(kk, W[2].w[0], W[2].w[1], W[2].w[2]), gz = sel3(kk, W[2].g[0], W[2].g[1], W[2].g[2]);
 #pragma unroll
-    for (int a = 0; a < 3; a++) q[a] = (int)A[(F_QX + a) * cap + slot];
-    const uint32_t tag = A[F_TAG * cap + slot];
-    const MatDev &M = s_mat[tag_mat(tag)];
-    float v[3], s[6];
+      for (int jj = 0; jj < 3; jj++) {
+        const float Aw = W[1].w[jj] * wz;
+        const float By = W[1].g[jj] * wz;
+        const float Bz = W[1].w[jj] * gz;
+        float S0[3] = {0, 0, 0}, S1[3] = {0, 0, 0}, Sa[3] = {0, 0, 0};
+        const int row = r0 + G2P_R * ((r1 + jj) + G2P_R * (r2 + kk));
 #pragma unroll
-    for (int a = 0; a < 3; a++) v[a] = f32(A[(F_VX + a) * cap + slot]);
-#pragma unroll
-    for (int a = 0; a < 6; a++) s[a] = f32(A[(F_S0 + a) * cap + slot]);
-    float rho = f32(A[F_RHO * cap + slot]);
-    const float mass = f32(A[F_MASS * cap + slot]);
-    bool bad = false;
-    int b[3];
-    Axis W[3];
-
-    if (UPDATE) {
-      const int out = stable_pos(d, keys_in, i, slot);
-      float e[6], p[6] = {0, 0, 0, 0, 0, 0}, epb = 0.0f, dmg = 0.0f, smax = 0.0f;
-#pragma unroll
-      for (int a = 0; a < 6; a++) e[a] = f32(A[(F_E0 + a) * cap + slot]);
-      const bool plastic = M.carry_plastic;
-      if (plastic) {
-#pragma unroll
-        for (int a = 0; a < 6; a++) p[a] = f32(A[(F_P0 + a) * cap + slot]);
-        epb = f32(A[F_EPB * cap + slot]);
-      }
-      if (M.carry_damage) {
-        dmg = f32(A[F_DMG * cap + slot]);
-        smax = f32(A[F_SMAX * cap + slot]);
-      }
-      // ---- G2P (Extrapolation.cpp:127-163, 288-367) ----
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
-        W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
-      }
-      float vg[3] = {0, 0, 0}, ap[3] = {0, 0, 0};
-      float Lx[3] = {0, 0, 0}, Ly[3] = {0, 0, 0}, Lz[3] = {0, 0, 0};  // d v_b / d x_a
-      int r0 = b[0] - o[0], r1 = b[1] - o[1], r2 = b[2] - o[2];
-      if ((unsigned)r0 >= (unsigned)TILE || (unsigned)r1 >= (unsigned)TILE || (unsigned)r2 >= (unsigned)TILE) {
-        // a particle listed under a tile it is not in: never expected; keep LDS reads in range
-        atomicOr(&d.ctrl->status, 2);
-        r0 = min(max(r0, 0), TILE - 1);
-        r1 = min(max(r1, 0), TILE - 1);
-        r2 = min(max(r2, 0), TILE - 1);
-      }
-#pragma unroll
-      for (int kk = 0; kk < 3; kk++)
-#pragma unroll
-        for (int jj = 0; jj < 3; jj++) {
-          const float Aw = W[1].w[jj] * W[2].w[kk];
-          const float By = W[1].g[jj] * W[2].w[kk];
-          const float Bz = W[1].w[jj] * W[2].g[kk];
-          float S0[3] = {0, 0, 0}, S1[3] = {0, 0, 0}, Sa[3] = {0, 0, 0};
-          const int row = r0 + G2P_R * ((r1 + jj) + G2P_R * (r2 + kk));
-#pragma unroll
-          for (int ii2 = 0; ii2 < 3; ii2++) {
-            const int n = row + ii2;
-            const float wx = W[0].w[ii2], gx = W[0].g[ii2];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-              const float vn = s_vel[c][n];
-              S0[c] += wx * vn;
-              S1[c] += gx * vn;
-              Sa[c] += wx * s_vel[3 + c][n];
-            }
-          }
+        for (int ii2 = 0; ii2 < 3; ii2++) {
+          const int n = row + ii2;
+          const float wx = W[0].w[ii2], gx = W[0].g[ii2];
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            vg[c] += Aw * S0[c];
-            ap[c] += Aw * Sa[c];
-            Lx[c] += Aw * S1[c];
-            Ly[c] += By * S0[c];
-            Lz[c] += Bz * S0[c];
+            const float vn = s_vel[c][n];
+            S0[c] += wx * vn;
+            S1[c] += gx * vn;
+            Sa[c] += wx * s_vel[3 + c][n];
           }
         }
-      float er[6];
-      er[0] = Lx[0];
-      er[1] = Ly[1];
-      er[2] = Lz[2];
-      er[3] = 0.5f * (Lx[1] + Ly[0]);
-      er[4] = 0.5f * (Ly[2] + Lz[1]);
-      er[5] = 0.5f * (Lx[2] + Lz[0]);
-      const float rho_dot = -rho * (Lx[0] + Ly[1] + Lz[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          vg[c] += Aw * S0[c];
+          ap[c] += Aw * Sa[c];
+          Lx[c] += Aw * S1[c];
+          Ly[c] += By * S0[c];
+          Lz[c] += Bz * S0[c];
+        }
+      }
+    }
+    float er[6];
+    er[0] = Lx[0];
+    er[1] = Ly[1];
+    er[2] = Lz[2];
+    er[3] = 0.5f * (Lx[1] + Ly[0]);
+    er[4] = 0.5f * (Ly[2] + Lz[1]);
+    er[5] = 0.5f * (Lx[2] + Lz[0]);
+    const float rho_dot = -rho * (Lx[0] + Ly[1] + Lz[2]);
 
-      const int bct = tag_bct(tag);
-      float bv[4] = {0, 0, 0, 0};
-      const int pid = (int)A[F_PID * cap + slot];
+    const int bct = (MASK & KM_PBC) ? tag_bct(tag) : 0;
+    float bv[4] = {0, 0, 0, 0};
+    const int pid = (int)A[F_PID * cap + slot];
+    if (MASK & KM_PBC) {
       if (bct) {
 #pragma unroll
         for (int k = 0; k < 4; k++) bv[k] = d.pbc_val[4 * (long long)pid + k];
         particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 4 (Solver.cpp:219-223)
       }
-      // ---- particle update (TimeIntegration.cpp:11-117) ----
-      const float dt = I.dt;
-      rho += rho_dot * dt;
-      if (!tag_hold(tag)) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-          v[a] = v[a] + (ap[a] - I.damping * v[a]) * dt;
-          q[a] += __float2int_rn(vg[a] * I.qscale);
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < 6; a++) e[a] += er[a] * dt;
-      if (!M.rigid) {
-        if (M.model == 1) cm_elastic(M, er, s, dt);
-        else if (M.model == 2) cm_newtonian(M, er, e, s, dt);
-        else if (M.model == 3) cm_perfect(M, er, s, p, epb, dt);
-        else if (M.model == 4) cm_drucker_prager(M, er, s, p, epb, dt);
-        if (M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
-      }
-      // periodic wrap with period L-2h (TimeIntegration.cpp:80-108)
+    }
+    // ---- particle update (TimeIntegration.cpp:11-117) ----
+    const float dt = I.dt;
+    rho += rho_dot * dt;
+    if (!tag_hold(tag)) {
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        if (q[a] > I.wrapq[a]) q[a] = I.oneq + q[a] % I.wrapq[a];
-        if (q[a] < I.oneq) q[a] += I.wrapq[a] - I.oneq;
+        v[a] = v[a] + (ap[a] - I.damping * v[a]) * dt;
+        q[a] += __float2int_rn(vg[a] * I.qscale);
       }
-      if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
-
-      // ---- store the new state at its sorted slot ----
+    }
+    float e[6];
 #pragma unroll
-      for (int a = 0; a < 3; a++) B[(F_QX + a) * cap + out] = (uint32_t)q[a];
+    for (int a = 0; a < 6; a++) e[a] = f32(A[(F_E0 + a) * cap + slot]) + er[a] * dt;
 #pragma unroll
-      for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
+    for (int a = 0; a < 6; a++) B[(F_E0 + a) * cap + out] = u32(e[a]);
+    if (!M.rigid) {
+      if ((MASK & KM_ELASTIC) && M.model == 1) cm_elastic(M, er, s, dt);
+      if ((MASK & KM_NEWTON) && M.model == 2) cm_newtonian(M, er, e, s, dt);
+      if ((MASK & (KM_PERFECT | KM_DP)) && M.carry_plastic) {
+        float p[6];
 #pragma unroll
-      for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
-#pragma unroll
-      for (int a = 0; a < 6; a++) B[(F_E0 + a) * cap + out] = u32(e[a]);
-      if (plastic) {
+        for (int a = 0; a < 6; a++) p[a] = f32(A[(F_P0 + a) * cap + slot]);
+        float epb = f32(A[F_EPB * cap + slot]);
+        if ((MASK & KM_PERFECT) && M.model == 3) cm_perfect(M, er, s, p, epb, dt);
+        if ((MASK & KM_DP) && M.model == 4) cm_drucker_prager(M, er, s, p, epb, dt);
 #pragma unroll
         for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = u32(p[a]);
         B[F_EPB * cap + out] = u32(epb);
       }
-      if (M.carry_damage) {
-        B[F_DMG * cap + out] = u32(dmg);
-        B[F_SMAX * cap + out] = u32(smax);
-      }
-      B[F_RHO * cap + out] = u32(rho);
-      B[F_MASS * cap + out] = u32(mass);
-      B[F_TAG * cap + out] = tag;
-      B[F_PID * cap + out] = (uint32_t)pid;
-
-      // ---- new binning key + histogram ----
-      bad = false;
-#pragma unroll
-      for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
-      const int knew = key_of(I, b);
-      keys_out[out] = knew;
-      d.rank[out] = atomicAdd(&d.count[knew], 1);
-      if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
-    } else {
-#pragma unroll
-      for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
-    }
-
-    // ---- P2G of the (new) state (Extrapolation.cpp:166-286) ----
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-      W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
-    P2GTerms T;
-    const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
-    T.m = mp;
-#pragma unroll
-    for (int a = 0; a < 3; a++) T.mv[a] = mp * v[a];
-    const float nV = -(mp / rho);
-#pragma unroll
-    for (int a = 0; a < 6; a++) T.vs[a] = nV * s[a];
-    const int r0 = b[0] - o[0] + 1, r1 = b[1] - o[1] + 1, r2 = b[2] - o[2] + 1;
-    const bool inside = (r0 >= 0 && r1 >= 0 && r2 >= 0 && r0 + 2 < P2G_R && r1 + 2 < P2G_R && r2 + 2 < P2G_R);
-    if (inside) {
-#pragma unroll
-      for (int kk = 0; kk < 3; kk++)
-#pragma unroll
-        for (int jj = 0; jj < 3; jj++) {
-          const float Aw = W[1].w[jj] * W[2].w[kk];
-          const float By = W[1].g[jj] * W[2].w[kk];
-          const float Bz = W[1].w[jj] * W[2].g[kk];
-          const int row = r0 + P2G_R * ((r1 + jj) + P2G_R * (r2 + kk));
-#pragma unroll
-          for (int ii2 = 0; ii2 < 3; ii2++) {
-            float tt[7];
-            node_terms(T, W[0].w[ii2] * Aw, W[0].g[ii2] * Aw, W[0].w[ii2] * By, W[0].w[ii2] * Bz, tt);
-#pragma unroll
-            for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], tt[f]);
-          }
+      if (MASK & KM_GK) {
+        if (M.carry_damage) {
+          float dmg = f32(A[F_DMG * cap + slot]), smax = f32(A[F_SMAX * cap + slot]);
+          if (M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
+          B[F_DMG * cap + out] = u32(dmg);
+          B[F_SMAX * cap + out] = u32(smax);
         }
+      }
     } else {
-      p2g_global(d, I, b, W, T);
-    }
-  }
-  __syncthreads();
-
-  // ---- flush the LDS tile: one set of global atomics per touched node ----
-  for (int n = tid; n < NODES_P2G; n += WG) {
-    const float m = s_acc[0][n];
-    if (m == 0.0f) continue;
-    const int g[3] = {o[0] - 1 + n % P2G_R, o[1] - 1 + (n / P2G_R) % P2G_R, o[2] - 1 + n / (P2G_R * P2G_R)};
-    if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
-    int c[3];
-    const long long ci = canon_index(I, g, c);
-    float *dst = d.acc + ci * NODE_W;
+      // rigid particles keep their stress, plastic and damage fields
+      if (M.carry_plastic) {
 #pragma unroll
-    for (int f = 0; f < 7; f++) unsafeAtomicAdd(dst + f, s_acc[f][n]);
-    flag_brick(d, I, c);
+        for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = A[(F_P0 + a) * cap + slot];
+        B[F_EPB * cap + out] = A[F_EPB * cap + slot];
+      }
+      if (M.carry_damage) {
+        B[F_DMG * cap + out] = A[F_DMG * cap + slot];
+        B[F_SMAX * cap + out] = A[F_SMAX * cap + slot];
+      }
+    }
+    if (!(MASK & (KM_PERFECT | KM_DP)) && M.carry_plastic && !M.rigid) {
+      // plastic fields uploaded for a model that never changes them
+#pragma unroll
+      for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = A[(F_P0 + a) * cap + slot];
+      B[F_EPB * cap + out] = A[F_EPB * cap + slot];
+    }
+    if (!(MASK & KM_GK) && M.carry_damage && !M.rigid) {
+      B[F_DMG * cap + out] = A[F_DMG * cap + slot];
+      B[F_SMAX * cap + out] = A[F_SMAX * cap + slot];
+    }
+    // periodic wrap with period L-2h (TimeIntegration.cpp:80-108)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (q[a] > I.wrapq[a]) q[a] = I.oneq + q[a] % I.wrapq[a];
+      if (q[a] < I.oneq) q[a] += I.wrapq[a] - I.oneq;
+    }
+    if (MASK & KM_PBC) {
+      if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+    }
+
+    // ---- store the new state at its sorted slot ----
+#pragma unroll
+    for (int a = 0; a < 3; a++) B[(F_QX + a) * cap + out] = (uint32_t)q[a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
+#pragma unroll
+    for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
+    B[F_RHO * cap + out] = u32(rho);
+    B[F_MASS * cap + out] = u32(mass);
+    B[F_TAG * cap + out] = tag;
+    B[F_PID * cap + out] = (uint32_t)pid;
+
+    // ---- new binning key + histogram ----
+    bad = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+    const int knew = key_of(I, b);
+    (par ? d.key[0] : d.key[1])[out] = knew;
+    d.rank[out] = atomicAdd(&d.count[knew], 1);
+    if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+  }
+
+  // ---- P2G of the (new) state (Extrapolation.cpp:166-286) ----
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+  P2GTerms T;
+  const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
+  T.m = mp;
+#pragma unroll
+  for (int a = 0; a < 3; a++) T.mv[a] = mp * v[a];
+  const float nV = -(mp / rho);
+#pragma unroll
+  for (int a = 0; a < 6; a++) T.vs[a] = nV * s[a];
+  const int r0 = b[0] - o[0] + 1, r1 = b[1] - o[1] + 1, r2 = b[2] - o[2] + 1;
+  const bool inside = (r0 >= 0 && r1 >= 0 && r2 >= 0 && r0 + 2 < P2G_R && r1 + 2 < P2G_R && r2 + 2 < P2G_R);
+  if (inside) {
+#pragma unroll 1
+    for (int kk = 0; kk < 3; kk++) {
+      const float wz = sel3(kk, W[2].w[0], W[2].w[1], W[2].w[2]), gz = sel3(kk, W[2].g[0], W[2].g[1], W[2].g[2]);
+#pragma unroll
+      for (int jj = 0; jj < 3; jj++) {
+        const float Aw = W[1].w[jj] * wz;
+        const float By = W[1].g[jj] * wz;
+        const float Bz = W[1].w[jj] * gz;
+        const int row = r0 + P2G_R * ((r1 + jj) + P2G_R * (r2 + kk));
+#pragma unroll
+        for (int ii2 = 0; ii2 < 3; ii2++) {
+          float tt[7];
+          node_terms(T, W[0].w[ii2] * Aw, W[0].g[ii2] * Aw, W[0].w[ii2] * By, W[0].w[ii2] * Bz, tt);
+#pragma unroll
+          for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], (double)tt[f]);
+        }
+      }
+    }
+  } else {
+    p2g_global(d.acc, d.brick_flag, &d.inst[inst], b[0], b[1], b[2], W[0], W[1], W[2], T);
+  }
+}
+
+// The fused particle kernel, persistent over the active tile list. UPDATE = false: P2G of
+// the current state only (first step after an upload). UPDATE = true: G2P + update +
+// constitutive + P2G of the new state.
+//
+// P2G accumulates in FP64 LDS (ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32
+// LDS atomic adds) over the tile's region [o-1, o+TILE+3)^3, then writes the region as
+// FP32 partial sums with plain stores and stamps the tile's epoch; the grid kernel gathers
+// the partials of the tiles covering each node, in a fixed order.
+//
+// Lane mapping CELL: lane l of every wave owns cell l of the 4x4x4 tile and the four waves
+// stride through that cell's particles (no two lanes of an LDS atomic hit one address).
+// !CELL: consecutive lanes take consecutive particles of the tile (coalesced state access).
+template <bool UPDATE, int MASK, bool CELL>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_g2p2g(DevPtrs d) {
+  static_assert(WG == 4 * TILE3, "one lane per cell, four waves per tile");
+  __shared__ float s_vel[6][NODES_G2P];
+  __shared__ double s_acc[7][NODES_P2G];
+  __shared__ MatDev s_mat[MAXMAT];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x;
+  const int grp = blockIdx.x & 7;  // dispatch XCD of this workgroup (round-robin)
+  const int nact = d.ctrl->active_tiles;
+  const int lo = (int)((long long)grp * nact / 8), hi = (int)((long long)(grp + 1) * nact / 8);
+  const int par = d.ctrl->parity;
+  const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
+
+  for (;;) {
+    if (tid == 0) {
+      const int j = lo + atomicAdd(&d.ctrl->next[grp], 1);
+      s_tile = (j < hi) ? d.active[j] : -1;
+    }
+    __syncthreads();
+    const int t = s_tile;
+    if (t < 0) break;
+    const int k0 = t * TILE3;
+    const int ii = d.tile_inst[t];
+    const InstDesc I = d.inst[ii];
+    const int lt = t - I.tile_base;
+    const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
+
+    for (int n = tid; n < 7 * NODES_P2G; n += WG) (&s_acc[0][0])[n] = 0.0;
+    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+    if (UPDATE) {
+      for (int n = tid; n < NODES_G2P; n += WG) {
+        const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
+        const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
+        float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+        if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
+          const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
+          const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
+          v0 = vp[0];
+          v1 = vp[1];
+        }
+        s_vel[0][n] = v0.x; s_vel[1][n] = v0.y; s_vel[2][n] = v0.z;
+        s_vel[3][n] = v0.w; s_vel[4][n] = v1.x; s_vel[5][n] = v1.y;
+      }
+    }
+    __syncthreads();
+
+    const double tl = UPDATE ? d.t_last[ii] : 0.0;
+    if (CELL) {
+      const int cell = tid & (TILE3 - 1);
+      const int seg_lo = d.offsets[k0 + cell], seg_hi = d.offsets[k0 + cell + 1];
+      for (int i = seg_lo + (tid >> 6); i < seg_hi; i += WG / TILE3)
+        particle_step<UPDATE, MASK>(d, I, o, ii, tl, i, seg_lo, seg_hi, s_vel, s_acc, s_mat, par);
+    } else {
+      const int p0 = d.offsets[k0], p1 = d.offsets[k0 + TILE3];
+      const int *keys_in = (par ? d.key[1] : d.key[0]);
+      for (int i = p0 + tid; i < p1; i += WG) {
+        const int k = UPDATE ? keys_in[d.tmp[i]] : 0;
+        const int seg_lo = UPDATE ? d.offsets[k] : 0, seg_hi = UPDATE ? d.offsets[k + 1] : 0;
+        particle_step<UPDATE, MASK>(d, I, o, ii, tl, i, seg_lo, seg_hi, s_vel, s_acc, s_mat, par);
+      }
+    }
+    __syncthreads();
+
+    // ---- flush: the region's sums as FP32 partials (plain stores), epoch, brick flags ----
+    float *pt = d.partial + (long long)t * PARTIAL_W;
+    for (int n = tid; n < PARTIAL_W; n += WG) pt[n] = (float)(&s_acc[0][0])[n];
+    for (int n = tid; n < NODES_P2G; n += WG) {
+      if (s_acc[0][n] == 0.0) continue;
+      const int g[3] = {o[0] - 1 + n % P2G_R, o[1] - 1 + (n / P2G_R) % P2G_R, o[2] - 1 + n / (P2G_R * P2G_R)};
+      if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
+      int c[3];
+      (void)canon_index(I, g, c);
+      flag_brick(d, I, c);
+    }
+    if (tid == 0) d.tile_epoch[t] = epoch_w;
+    __syncthreads();
   }
 }
 
@@ -866,7 +1020,7 @@ __global__ __launch_bounds__(WG) void k_audit(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
   double r[AUD] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (s < d.np) {
-    const uint32_t *A = d.pbuf[d.ctrl->parity];
+    const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
     const long long cap = d.cap;
     const double m = f32(A[F_MASS * cap + s]);
     double v[3], ss = 0;
@@ -939,15 +1093,29 @@ void launch_scan_final(const DevPtrs &d, void *st) {
 void launch_place(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_place, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
-void launch_p2g_only(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_g2p2g<false>, dim3(d.ntiles), dim3(WG), 0, S(st), d);
+// Smallest built model-set specialization covering the context's mask.
+template <bool U, bool CELL>
+void launch_fused_map(const DevPtrs &d, hipStream_t st) {
+  const int m = d.kmask;
+  const dim3 g(d.persist_wgs), b(WG);
+  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP, CELL>), g, b, 0, st, d);
+  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC, CELL>), g, b, 0, st, d);
+  else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT, CELL>), g, b, 0, st, d);
+  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC | KM_GK, CELL>), g, b, 0, st, d);
+  else hipLaunchKernelGGL((k_g2p2g<U, KM_ALL, CELL>), g, b, 0, st, d);
 }
+
+template <bool U>
+void launch_fused(const DevPtrs &d, hipStream_t st) {
+  if (d.cell_lanes) launch_fused_map<U, true>(d, st);
+  else launch_fused_map<U, false>(d, st);
+}
+
+void launch_p2g_only(const DevPtrs &d, void *st) { launch_fused<false>(d, S(st)); }
 void launch_grid(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
 }
-void launch_g2p2g(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_g2p2g<true>, dim3(d.ntiles), dim3(WG), 0, S(st), d);
-}
+void launch_g2p2g(const DevPtrs &d, void *st) { launch_fused<true>(d, S(st)); }
 void launch_advance(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_advance, dim3(blocks(d.ninst, WG)), dim3(WG), 0, S(st), d);
 }
