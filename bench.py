@@ -12,7 +12,7 @@ Prints ONE JSON line on stdout (everything else goes to stderr). `value` is whol
 particle-steps/s with the state resident in HBM; `roofline` is the dominant kernel's
 algorithmic bytes per launch over its average duration, timed with HIP events on the
 context's own stream during the timed region; `cpu_baseline` times the reference's
-iterate_CPU (compiled from its sources into oracle/_ref) on a 20-step sample of the same
+iterate_CPU (compiled from its sources into oracle/_ref) on a 200-step sample of the same
 workload on this host's cores.
 """
 from __future__ import annotations
@@ -38,6 +38,21 @@ B_ALG_STEP = 241
 # The fused particle kernel carries all of it except the binning offset read and
 # permutation write (8 B), which the scan/place kernels carry.
 B_ALG_G2P2G = B_ALG_STEP - 8
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "g2p2g_dp_column_1M.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration), or None."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        blob = text.split("```json")[-1].split("```")[0]
+        rec = json.loads(blob)
+        return int(rec["traffic_bytes"]) if rec.get("kernel") == kernel else None
+    except (OSError, ValueError, KeyError, IndexError):
+        return None
 
 
 def log(*a):
@@ -66,16 +81,36 @@ def cpu_baseline(sample_steps: int, warm: int) -> dict:
                       f"OMP_NUM_THREADS={cores}, {dt:.1f} s"}
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
+    """A plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters
+    per dispatch, divided by nbytes, give the scale applied to the fused kernel's counters."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies of {nbytes} bytes")
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        return copy_calibration()
 
     if args.cpu_baseline_only:
         print(json.dumps(cpu_baseline(args.cpu_steps, 2)))
@@ -121,7 +156,7 @@ def main():
     bytes_per_launch = (B_ALG_G2P2G if dom == "g2p2g" else B_ALG_STEP) * sc.n
     achieved = bytes_per_launch / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch}
 
     cpu = None

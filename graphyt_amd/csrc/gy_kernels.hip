@@ -1,15 +1,16 @@
 // gy_kernels.hip — HIP kernels (gfx950) of the MPM time step.
 //
 // One step is six launches on one stream:
-//   k_grid      nodal solve on the active canonical bricks: v = p/m, a = f/m + g,
-//               body-1 kick (half at step 1), BCs, periodic fan-out to every mirror node,
-//               and the P2G accumulators cleared for the next step
+//   k_grid      nodal solve on the active canonical bricks: gather the P2G tile partials
+//               of every node of the periodic class, v = p/m, a = f/m + g, body-1 kick
+//               (half at step 1), BCs, fan-out to every mirror node
 //               (Grid.cpp:9-245, 310-603, 862-1094)
-//   k_g2p2g     per cell tile: G2P from LDS-staged node velocities, particle update,
-//               constitutive/damage models, periodic wrap, new cell key + histogram, and
-//               P2G of the updated particle into an LDS tile with a one-node halo, flushed
-//               with a few global FP32 atomics (Extrapolation.cpp:31-367,
-//               TimeIntegration.cpp:11-296, ConstitutiveModels.cpp, DamageModels.cpp)
+//   k_g2p2g     persistent over the active cell tiles: G2P from LDS-staged node
+//               velocities, particle update, constitutive/damage models, periodic wrap,
+//               new cell key + histogram, and P2G of the updated particle into an FP64 LDS
+//               region with a one-node halo, written out as per-tile FP32 partial sums
+//               with plain stores (Extrapolation.cpp:31-367, TimeIntegration.cpp:11-296,
+//               ConstitutiveModels.cpp, DamageModels.cpp)
 //   k_advance   step/parity/time bookkeeping on the device
 //   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets)
 //   k_place     key-grouped slot list; the stable within-cell order (ascending slot) is
