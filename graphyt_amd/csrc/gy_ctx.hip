@@ -282,7 +282,6 @@ int alloc_grid(gy_ctx *c) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
     d.persist_wgs = 8 * ((cus * 3 + 7) / 8);  // three resident 256-thread workgroups per CU
   }
-  d.cell_lanes = 1;
   if ((rc = dalloc(c, &d.inst, c->inst.size()))) return rc;
   if ((rc = dalloc(c, &d.mats, c->mats.size()))) return rc;
   if ((rc = dalloc(c, &d.ctrl, 1))) return rc;
@@ -683,14 +682,15 @@ int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *cou
   if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess) return GY_ERR_HIP;
   DevPtrs &d = tmp.d;
   d.np = (int)n;
-  d.nkeys = (int)nkeys;
+  // the scan kernels take whole tiles of keys: pad the key space (padded keys stay empty)
+  d.nkeys = (int)((nkeys + TILE3 - 1) / TILE3 * TILE3);
   d.nblocks_scan = (int)((nkeys + scan_items_per_block() - 1) / scan_items_per_block());
   int rc = GY_OK;
   if ((rc = dalloc(&tmp, &d.key[0], (size_t)n)) || (rc = dalloc(&tmp, &d.rank, (size_t)n)) ||
-      (rc = dalloc(&tmp, &d.tmp, (size_t)n)) || (rc = dalloc(&tmp, &d.count, (size_t)nkeys)) ||
-      (rc = dalloc(&tmp, &d.offsets, (size_t)nkeys + 1)) || (rc = dalloc(&tmp, &d.blocksum, (size_t)d.nblocks_scan)) ||
+      (rc = dalloc(&tmp, &d.tmp, (size_t)n)) || (rc = dalloc(&tmp, &d.count, (size_t)d.nkeys)) ||
+      (rc = dalloc(&tmp, &d.offsets, (size_t)d.nkeys + 1)) || (rc = dalloc(&tmp, &d.blocksum, (size_t)d.nblocks_scan)) ||
       (rc = dalloc(&tmp, &d.blocktiles, (size_t)d.nblocks_scan)) ||
-      (rc = dalloc(&tmp, &d.active, (size_t)(nkeys + TILE3 - 1) / TILE3 + 1)) || (rc = dalloc(&tmp, &d.ctrl, 1))) {
+      (rc = dalloc(&tmp, &d.active, (size_t)d.nkeys / TILE3 + 1)) || (rc = dalloc(&tmp, &d.ctrl, 1))) {
     rc = GY_ERR_HIP;
   } else {
     int *dord = nullptr;
@@ -729,7 +729,6 @@ int gy_set_option(gy_ctx *c, int32_t option, int64_t value) {
     case GY_OPT_USE_GRAPH: c->use_graph = value != 0; if (!c->use_graph) drop_graph(c); return GY_OK;
     case GY_OPT_PROFILE: c->profile = value != 0; return GY_OK;
     case GY_OPT_TILE: return value == TILE ? GY_OK : fail(c, GY_ERR_UNSUPPORTED, "tile edge %lld not built", (long long)value);
-    case GY_OPT_CELL_LANES: c->d.cell_lanes = value != 0; drop_graph(c); return GY_OK;
     case GY_OPT_PERSIST_WGS:
       if (value < 8 || value > 65536) return fail(c, GY_ERR_INVALID, "persistent workgroups must be in [8, 65536]");
       c->d.persist_wgs = (int)(8 * ((value + 7) / 8));

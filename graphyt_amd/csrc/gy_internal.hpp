@@ -148,7 +148,6 @@ struct DevPtrs {
   int nblocks_scan;
   int kmask;             // model-set bits of the fused kernel specialization
   int persist_wgs;       // workgroups of the persistent particle kernel (multiple of 8)
-  int cell_lanes;        // 1: lane = cell of the tile; 0: lane = consecutive particle
 };
 
 // Kernel identifiers for per-kernel timing.

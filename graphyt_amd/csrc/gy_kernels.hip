@@ -428,13 +428,26 @@ __global__ __launch_bounds__(WG) void k_hist(DevPtrs d) {
 // threads (16 keys each) hold one tile, so tile occupancy is a 4-lane shuffle sum.
 static_assert(TILE3 == 4 * SCAN_ITEMS, "a tile is four scan threads");
 
+// The key count is a multiple of TILE3, so a thread's SCAN_ITEMS keys are all present or
+// all absent; they are read and written as four 16-byte vectors.
+static_assert(SCAN_ITEMS == 16, "four int4 per scan thread");
+__device__ __forceinline__ void load16(const int *p, int c[16]) {
+  const int4 *v = reinterpret_cast<const int4 *>(p);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int4 x = v[q];
+    c[4 * q] = x.x; c[4 * q + 1] = x.y; c[4 * q + 2] = x.z; c[4 * q + 3] = x.w;
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d) {
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
+  if (base < d.nkeys) {
+    int c[SCAN_ITEMS];
+    load16(d.count + base, c);
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; j++) {
-    const long long k = base + j;
-    if (k < d.nkeys) s += d.count[k];
+    for (int j = 0; j < SCAN_ITEMS; j++) s += c[j];
   }
   int ts = s + __shfl_xor(s, 1, 64);
   ts += __shfl_xor(ts, 2, 64);
@@ -457,24 +470,30 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   pre = block_sum(pre);
   pret = block_sum(pret);
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+  const bool have = base < d.nkeys;
   int c[SCAN_ITEMS];
   int s = 0;
+  if (have) load16(d.count + base, c);
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; j++) {
-    const long long k = base + j;
-    c[j] = (k < d.nkeys) ? d.count[k] : 0;
+    if (!have) c[j] = 0;
     s += c[j];
   }
   int tot;
   int run = pre + block_exclusive_scan(s, tot);
+  if (have) {
+    int4 *ov = reinterpret_cast<int4 *>(d.offsets + base);
+    int4 *cv = reinterpret_cast<int4 *>(d.count + base);
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; j++) {
-    const long long k = base + j;
-    if (k < d.nkeys) {
-      d.offsets[k] = run;
-      d.count[k] = 0;
+    for (int q = 0; q < 4; q++) {
+      int4 x;
+      x.x = run; run += c[4 * q];
+      x.y = run; run += c[4 * q + 1];
+      x.z = run; run += c[4 * q + 2];
+      x.w = run; run += c[4 * q + 3];
+      ov[q] = x;
+      cv[q] = make_int4(0, 0, 0, 0);
     }
-    run += c[j];
   }
   // compacted active tile list in ascending tile order
   int ts = s + __shfl_xor(s, 1, 64);
@@ -690,18 +709,51 @@ __device__ __noinline__ void p2g_global(float *acc, uint8_t *brick_flag, const I
 constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32;
 constexpr int KM_ALL = 63;
 
-// One particle of the fused kernel: G2P + update + constitutive + store (UPDATE), then P2G
-// of the (new) state into the tile's FP64 LDS accumulators. `i` is the particle's index in
-// the key-grouped slot list and [seg_lo, seg_hi) its cell's segment there.
+// The fused kernel works through a tile's entries in chunks of CHUNK list entries.
+constexpr int CHUNK = WG;
+// FP64 LDS accumulator region: P2G_R^3 nodes, x fastest, z stride padded to 68 doubles
+// (68 = 4 mod 32) so the 64 lanes of one atomic, spread over different cells, fall on
+// evenly used banks.
+constexpr int ACC_YS = P2G_R;
+constexpr int ACC_ZS = P2G_R * P2G_R + 4;
+constexpr int ACC_N = ACC_ZS * P2G_R;
+// P2G record of one particle, handed from the update phase to the scatter phase in LDS:
+// fixed-point position (3), m, m*v (3), -V*sigma (6).
+constexpr int NREC = 13;
+
+// Rank of a lane's key among the equal keys of its wave, with one global atomic per run
+// of equal keys (lanes hold consecutive entries of a cell-grouped list, so equal keys
+// are adjacent). key < 0: the lane takes no part (only trailing lanes may do so).
+__device__ __forceinline__ int wave_hist_rank(int *count, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = key >= 0 && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valid = __ballot(key >= 0);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  int base = 0;
+  if (head) {
+    const unsigned long long later = heads & ~upto;
+    const int vend = 64 - __clzll((long long)valid);
+    const int end = later ? __ffsll((long long)later) - 1 : vend;
+    base = atomicAdd(&count[key], end - lane);
+  }
+  const unsigned long long mine = heads & upto;
+  const int hl = mine ? 63 - __clzll((long long)mine) : lane;
+  base = __shfl(base, hl, 64);
+  return base + lane - hl;
+}
+
+// Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
+// constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
+// state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
 template <bool UPDATE, int MASK>
-__device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
-                                              double tl, int i, int seg_lo, int seg_hi,
-                                              float (*s_vel)[NODES_G2P], double (*s_acc)[NODES_P2G],
-                                              const MatDev *s_mat, int par) {
+__device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
+                                               int i, int slot, float (*s_vel)[NODES_G2P], const MatDev *s_mat,
+                                               int par, float (*s_rec)[CHUNK], int j) {
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
   const long long cap = d.cap;
-  const int slot = d.tmp[i];
   int q[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) q[a] = (int)A[(F_QX + a) * cap + slot];
@@ -714,14 +766,13 @@ __device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &
   for (int a = 0; a < 6; a++) s[a] = f32(A[(F_S0 + a) * cap + slot]);
   float rho = f32(A[F_RHO * cap + slot]);
   const float mass = f32(A[F_MASS * cap + slot]);
-  bool bad = false;
-  int b[3];
-  Axis W[3];
+  int knew = -1;
 
   if (UPDATE) {
-    // stable counting-sort slot: ascending slot order inside the cell
-    int out = seg_lo;
-    for (int j = seg_lo; j < seg_hi; j++) out += (d.tmp[j] < slot) ? 1 : 0;
+    const int out = i;
+    bool bad = false;
+    int b[3];
+    Axis W[3];
     // ---- G2P (Extrapolation.cpp:127-163, 288-367) ----
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -858,7 +909,7 @@ __device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &
       if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
     }
 
-    // ---- store the new state at its sorted slot ----
+    // ---- store the new state at its sorted position ----
 #pragma unroll
     for (int a = 0; a < 3; a++) B[(F_QX + a) * cap + out] = (uint32_t)q[a];
 #pragma unroll
@@ -870,31 +921,47 @@ __device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &
     B[F_TAG * cap + out] = tag;
     B[F_PID * cap + out] = (uint32_t)pid;
 
-    // ---- new binning key + histogram ----
+    // ---- new binning key ----
     bad = false;
 #pragma unroll
     for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
-    const int knew = key_of(I, b);
+    knew = key_of(I, b);
     (par ? d.key[0] : d.key[1])[out] = knew;
-    d.rank[out] = atomicAdd(&d.count[knew], 1);
     if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
-  } else {
-#pragma unroll
-    for (int a = 0; a < 3; a++) b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
   }
 
-  // ---- P2G of the (new) state (Extrapolation.cpp:166-286) ----
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-    W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
-  P2GTerms T;
+  // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
-  T.m = mp;
-#pragma unroll
-  for (int a = 0; a < 3; a++) T.mv[a] = mp * v[a];
   const float nV = -(mp / rho);
 #pragma unroll
-  for (int a = 0; a < 6; a++) T.vs[a] = nV * s[a];
+  for (int a = 0; a < 3; a++) s_rec[a][j] = __int_as_float(q[a]);
+  s_rec[3][j] = mp;
+#pragma unroll
+  for (int a = 0; a < 3; a++) s_rec[4 + a][j] = mp * v[a];
+#pragma unroll
+  for (int a = 0; a < 6; a++) s_rec[7 + a][j] = nV * s[a];
+  return knew;
+}
+
+// Phase B for one record: GIMP weights of its position and the 27-node scatter of m, m*v
+// and -V*sigma.gradN into the tile's FP64 LDS region (Extrapolation.cpp:166-286).
+__device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
+                                                 const float (*s_rec)[CHUNK], int j, double (*s_acc)[ACC_N]) {
+  int q[3], b[3];
+  bool bad = false;
+  Axis W[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q[a] = __float_as_int(s_rec[a][j]);
+    b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+    W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+  }
+  P2GTerms T;
+  T.m = s_rec[3][j];
+#pragma unroll
+  for (int a = 0; a < 3; a++) T.mv[a] = s_rec[4 + a][j];
+#pragma unroll
+  for (int a = 0; a < 6; a++) T.vs[a] = s_rec[7 + a][j];
   const int r0 = b[0] - o[0] + 1, r1 = b[1] - o[1] + 1, r2 = b[2] - o[2] + 1;
   const bool inside = (r0 >= 0 && r1 >= 0 && r2 >= 0 && r0 + 2 < P2G_R && r1 + 2 < P2G_R && r2 + 2 < P2G_R);
   if (inside) {
@@ -906,7 +973,7 @@ __device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &
         const float Aw = W[1].w[jj] * wz;
         const float By = W[1].g[jj] * wz;
         const float Bz = W[1].w[jj] * gz;
-        const int row = r0 + P2G_R * ((r1 + jj) + P2G_R * (r2 + kk));
+        const int row = r0 + ACC_YS * (r1 + jj) + ACC_ZS * (r2 + kk);
 #pragma unroll
         for (int ii2 = 0; ii2 < 3; ii2++) {
           float tt[7];
@@ -925,19 +992,26 @@ __device__ __forceinline__ void particle_step(const DevPtrs &d, const InstDesc &
 // the current state only (first step after an upload). UPDATE = true: G2P + update +
 // constitutive + P2G of the new state.
 //
-// P2G accumulates in FP64 LDS (ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32
-// LDS atomic adds) over the tile's region [o-1, o+TILE+3)^3, then writes the region as
-// FP32 partial sums with plain stores and stamps the tile's epoch; the grid kernel gathers
-// the partials of the tiles covering each node, in a fixed order.
-//
-// Lane mapping CELL: lane l of every wave owns cell l of the 4x4x4 tile and the four waves
-// stride through that cell's particles (no two lanes of an LDS atomic hit one address).
-// !CELL: consecutive lanes take consecutive particles of the tile (coalesced state access).
-template <bool UPDATE, int MASK, bool CELL>
+// A tile's list entries are taken in chunks of CHUNK, in two phases per chunk:
+//  A. lane = consecutive entry (coalesced state reads and writes): G2P from LDS-staged
+//     node velocities, update, constitutive/damage models, store at the entry's sorted
+//     position, new key with one histogram atomic per run of equal keys, and the P2G
+//     record into LDS;
+//  B. lane = entry in rank-major order (the r-th particles of every cell first), so the
+//     lanes of one LDS atomic belong to different cells and hit different nodes: 27-node
+//     scatter into the FP64 LDS region [o-1, o+TILE+3)^3.
+// ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32 LDS atomic adds. After the last
+// chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
+// is stamped; the grid kernel gathers the partials of the tiles covering each node in a
+// fixed order.
+template <bool UPDATE, int MASK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_g2p2g(DevPtrs d) {
-  static_assert(WG == 4 * TILE3, "one lane per cell, four waves per tile");
+  static_assert(WG == 4 * TILE3, "four waves per tile");
   __shared__ float s_vel[6][NODES_G2P];
-  __shared__ double s_acc[7][NODES_P2G];
+  __shared__ __align__(16) double s_acc[7][ACC_N];
+  __shared__ float s_rec[NREC][CHUNK];
+  __shared__ unsigned short s_list[CHUNK];
+  __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
@@ -949,8 +1023,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
 
   for (;;) {
     if (tid == 0) {
-      const int j = lo + atomicAdd(&d.ctrl->next[grp], 1);
-      s_tile = (j < hi) ? d.active[j] : -1;
+      const int jt = lo + atomicAdd(&d.ctrl->next[grp], 1);
+      s_tile = (jt < hi) ? d.active[jt] : -1;
     }
     __syncthreads();
     const int t = s_tile;
@@ -961,7 +1035,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int lt = t - I.tile_base;
     const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
 
-    for (int n = tid; n < 7 * NODES_P2G; n += WG) (&s_acc[0][0])[n] = 0.0;
+    {
+      double2 *z = reinterpret_cast<double2 *>(&s_acc[0][0]);
+      for (int n = tid; n < 7 * ACC_N / 2; n += WG) z[n] = make_double2(0.0, 0.0);
+    }
+    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
     if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
     if (UPDATE) {
       for (int n = tid; n < NODES_G2P; n += WG) {
@@ -981,28 +1059,47 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __syncthreads();
 
     const double tl = UPDATE ? d.t_last[ii] : 0.0;
-    if (CELL) {
-      const int cell = tid & (TILE3 - 1);
-      const int seg_lo = d.offsets[k0 + cell], seg_hi = d.offsets[k0 + cell + 1];
-      for (int i = seg_lo + (tid >> 6); i < seg_hi; i += WG / TILE3)
-        particle_step<UPDATE, MASK>(d, I, o, ii, tl, i, seg_lo, seg_hi, s_vel, s_acc, s_mat, par);
-    } else {
-      const int p0 = d.offsets[k0], p1 = d.offsets[k0 + TILE3];
-      const int *keys_in = (par ? d.key[1] : d.key[0]);
-      for (int i = p0 + tid; i < p1; i += WG) {
-        const int k = UPDATE ? keys_in[d.tmp[i]] : 0;
-        const int seg_lo = UPDATE ? d.offsets[k] : 0, seg_hi = UPDATE ? d.offsets[k + 1] : 0;
-        particle_step<UPDATE, MASK>(d, I, o, ii, tl, i, seg_lo, seg_hi, s_vel, s_acc, s_mat, par);
+    const int p0 = s_off[0], p1 = s_off[TILE3];
+    for (int c0 = p0; c0 < p1; c0 += CHUNK) {
+      const int c1 = min(c0 + CHUNK, p1), nc = c1 - c0;
+      // rank-major order of this chunk's entries (wave 0, lane = cell of the tile)
+      if (tid < TILE3) {
+        const int elo = max(s_off[tid], c0), ehi = min(s_off[tid + 1], c1);
+        const int n = max(ehi - elo, 0);
+        int maxn = n;
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) maxn = max(maxn, __shfl_xor(maxn, w, 64));
+        const unsigned long long below = (1ull << tid) - 1ull;
+        int S = 0;
+        for (int r = 0; r < maxn; r++) {
+          const unsigned long long m = __ballot(n > r);
+          if (n > r) s_list[S + __popcll(m & below)] = (unsigned short)(elo - c0 + r);
+          S += __popcll(m);
+        }
       }
+      // phase A
+      const int i = c0 + tid;
+      int knew = -1;
+      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, s_rec, tid);
+      if (UPDATE) {
+        const int r = wave_hist_rank(d.count, knew);
+        if (knew >= 0) d.rank[i] = r;
+      }
+      __syncthreads();
+      // phase B
+      if (tid < nc) particle_scatter(d, I, o, ii, s_rec, s_list[tid], s_acc);
+      __syncthreads();
     }
-    __syncthreads();
 
     // ---- flush: the region's sums as FP32 partials (plain stores), epoch, brick flags ----
     float *pt = d.partial + (long long)t * PARTIAL_W;
-    for (int n = tid; n < PARTIAL_W; n += WG) pt[n] = (float)(&s_acc[0][0])[n];
-    for (int n = tid; n < NODES_P2G; n += WG) {
-      if (s_acc[0][n] == 0.0) continue;
-      const int g[3] = {o[0] - 1 + n % P2G_R, o[1] - 1 + (n / P2G_R) % P2G_R, o[2] - 1 + n / (P2G_R * P2G_R)};
+    for (int n = tid; n < P2G_NODES; n += WG) {
+      const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
+      const int a = x + ACC_YS * y + ACC_ZS * z;
+#pragma unroll
+      for (int f = 0; f < 7; f++) pt[f * P2G_NODES + n] = (float)s_acc[f][a];
+      if (s_acc[0][a] == 0.0) continue;
+      const int g[3] = {o[0] - 1 + x, o[1] - 1 + y, o[2] - 1 + z};
       if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
       int c[3];
       (void)canon_index(I, g, c);
@@ -1095,21 +1192,15 @@ void launch_place(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_place, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
 // Smallest built model-set specialization covering the context's mask.
-template <bool U, bool CELL>
-void launch_fused_map(const DevPtrs &d, hipStream_t st) {
-  const int m = d.kmask;
-  const dim3 g(d.persist_wgs), b(WG);
-  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP, CELL>), g, b, 0, st, d);
-  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC, CELL>), g, b, 0, st, d);
-  else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT, CELL>), g, b, 0, st, d);
-  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC | KM_GK, CELL>), g, b, 0, st, d);
-  else hipLaunchKernelGGL((k_g2p2g<U, KM_ALL, CELL>), g, b, 0, st, d);
-}
-
 template <bool U>
 void launch_fused(const DevPtrs &d, hipStream_t st) {
-  if (d.cell_lanes) launch_fused_map<U, true>(d, st);
-  else launch_fused_map<U, false>(d, st);
+  const int m = d.kmask;
+  const dim3 g(d.persist_wgs), b(WG);
+  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP>), g, b, 0, st, d);
+  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC>), g, b, 0, st, d);
+  else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT>), g, b, 0, st, d);
+  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC | KM_GK>), g, b, 0, st, d);
+  else hipLaunchKernelGGL((k_g2p2g<U, KM_ALL>), g, b, 0, st, d);
 }
 
 void launch_p2g_only(const DevPtrs &d, void *st) { launch_fused<false>(d, S(st)); }

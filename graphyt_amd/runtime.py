@@ -45,7 +45,7 @@ SIGNATURES = {
     "gy_destroy": (None, [_P]),
 }
 
-OPT_USE_GRAPH, OPT_PROFILE, OPT_TILE, OPT_CELL_LANES, OPT_PERSIST_WGS = 1, 2, 3, 4, 5
+OPT_USE_GRAPH, OPT_PROFILE, OPT_TILE, OPT_PERSIST_WGS = 1, 2, 3, 5
 
 _lib = None
 

@@ -193,7 +193,7 @@ int  gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys,
 #define GY_OPT_USE_GRAPH   1   /* replay gy_iterate_n through a captured hipGraph (default 1) */
 #define GY_OPT_PROFILE     2   /* record HIP events around every kernel (default 0)          */
 #define GY_OPT_TILE        3   /* cell-tile edge of the particle kernels (4 is built)         */
-#define GY_OPT_CELL_LANES  4   /* 1: lane = cell of the tile, 0: lane = particle (default 1)  */
+/* option 4 is not assigned */
 #define GY_OPT_PERSIST_WGS 5   /* workgroups of the persistent particle kernel (default 3/CU) */
 int  gy_set_option(gy_ctx *ctx, int32_t option, int64_t value);
 
