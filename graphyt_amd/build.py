@@ -1,4 +1,6 @@
-"""Build the gfx950 shared library libgraphyt_amd.so in-tree with hipcc.
+"""Build the gfx950 shared library libgraphyt_amd.so in-tree with hipcc, and the C++
+host module graphyt_amd/graphyt*.so (the graphyt Python surface over that library) with
+g++ and pybind11.
 
     python -m graphyt_amd.build            # compile (always)
 
@@ -10,6 +12,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import sysconfig
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -45,5 +48,26 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host(verbose: bool = False) -> str:
+    """C++ host classes + pybind11 binding -> graphyt_amd/graphyt<EXT_SUFFIX>."""
+    import pybind11
+    src = os.path.join(PKG, "host", "graphyt_module.cpp")
+    out = os.path.join(PKG, "graphyt" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"], "-I", os.path.join(ROOT, "include"),
+           src, "-o", out + ".tmp", "-L", LIBDIR, "-lgraphyt_amd", "-Wl,-rpath,$ORIGIN/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host module build failed:\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(verbose: bool = False) -> list:
+    return [build(verbose), build_host(verbose)]
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print("\n".join(build_all(verbose="-v" in sys.argv)))

@@ -193,6 +193,9 @@ int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, con
     if (std::fabs(r - std::round(r)) > 1e-9 * std::max(1.0, r))
       return fail(c, GY_ERR_INVALID, "L[%d]/cellsize = %.12g is not an integer (Nodes.cpp:12 vs 71)", a, r);
     I.n[a] = (int)(1 + (g->L[a] / h));
+    if (I.n[a] != (int)std::llround(1 + r))
+      return fail(c, GY_ERR_INVALID, "L[%d]/cellsize = %.17g truncates to %d nodes (Nodes.cpp:71-73); lengthen L by an ulp",
+                  a, r, I.n[a]);
     if (I.n[a] < 6) return fail(c, GY_ERR_INVALID, "need at least 6 nodes per axis for the periodic pairs");
   }
   if (!(p->psep > 0) || p->psep > h) return fail(c, GY_ERR_UNSUPPORTED, "psep must be in (0, cellsize]");
@@ -276,6 +279,8 @@ int alloc_grid(gy_ctx *c) {
   if ((rc = dalloc(c, &d.blocktiles, (size_t)d.nblocks_scan))) return rc;
   if ((rc = dalloc(c, &d.active, (size_t)tile_base + 1))) return rc;
   if ((rc = dalloc(c, &d.tile_epoch, (size_t)tile_base))) return rc;
+  if ((rc = dalloc(c, &d.halo_epoch, (size_t)tile_base))) return rc;
+  if ((rc = dalloc(c, &d.pcore, (size_t)tile_base * CORE_W))) return rc;
   if ((rc = dalloc(c, &d.partial, (size_t)tile_base * PARTIAL_W))) return rc;
   {
     int cus = 0;
@@ -361,29 +366,44 @@ const char *gy_version(void) { return "graphyt_amd 0.1 (gfx950)"; }
 
 const char *gy_last_error(const gy_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-int gy_create(gy_ctx **out, const gy_grid *grid, const gy_material *mats, int32_t nmat, const gy_params *params) {
+static int create_impl(gy_ctx **out, int32_t ninst, const gy_grid *grids, const gy_material *mats,
+                       const int32_t *nmat, const gy_params *params) {
   if (!out) return GY_ERR_INVALID;
   *out = nullptr;
+  if (ninst <= 0 || !grids || !mats || !nmat || !params) return GY_ERR_INVALID;
   gy_ctx *c = new gy_ctx();
   int rc;
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return GY_ERR_HIP;
   }
-  c->ih.resize(1);
-  c->inst.resize(1);
-  memset(&c->inst[0], 0, sizeof(InstDesc));
-  if ((rc = describe(c, grid, mats, nmat, params, c->ih[0], c->inst[0])) || (rc = alloc_grid(c))) {
-    *out = c;  // keep the context so gy_last_error can report
-    return rc;
+  *out = c;  // kept on failure so gy_last_error can report
+  c->ih.resize(ninst);
+  c->inst.resize(ninst);
+  int moff = 0;
+  for (int i = 0; i < ninst; i++) {
+    memset(&c->inst[i], 0, sizeof(InstDesc));
+    if ((rc = describe(c, &grids[i], mats + moff, nmat[i], &params[i], c->ih[i], c->inst[i]))) {
+      if (ninst > 1) c->err = "instance " + std::to_string(i) + ": " + c->err;
+      return rc;
+    }
+    moff += nmat[i];
   }
-  *out = c;
-  return GY_OK;
+  return alloc_grid(c);
 }
 
-int gy_set_node_bc(gy_ctx *c, int32_t n_nodes, const int32_t *type, const double *val) {
-  if (!c || !type) return GY_ERR_INVALID;
-  if (n_nodes != c->nnodes) return fail(c, GY_ERR_INVALID, "expected %lld node BCs, got %d", c->nnodes, n_nodes);
+int gy_create(gy_ctx **out, const gy_grid *grid, const gy_material *mats, int32_t nmat, const gy_params *params) {
+  return create_impl(out, 1, grid, mats, &nmat, params);
+}
+
+int gy_create_batch(gy_ctx **out, int32_t ninst, const gy_grid *grids, const gy_material *mats,
+                    const int32_t *nmat, const gy_params *params) {
+  return create_impl(out, ninst, grids, mats, nmat, params);
+}
+
+int gy_instance_count(const gy_ctx *c) { return c ? (int)c->inst.size() : 0; }
+
+static int node_bc_impl(gy_ctx *c, long long base, int32_t n_nodes, const int32_t *type, const double *val) {
   std::vector<uint8_t> t(n_nodes);
   std::vector<float> v(2 * (size_t)n_nodes, 0.f);
   for (int i = 0; i < n_nodes; i++) {
@@ -394,28 +414,51 @@ int gy_set_node_bc(gy_ctx *c, int32_t n_nodes, const int32_t *type, const double
       v[2 * i + 1] = (float)val[4 * i + 1];
     }
   }
-  HIPCHECK(c, hipMemcpy(c->d.bc_type, t.data(), t.size(), hipMemcpyHostToDevice));
-  HIPCHECK(c, hipMemcpy(c->d.bc_val, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->d.bc_type + base, t.data(), t.size(), hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->d.bc_val + 2 * base, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
   return GY_OK;
 }
 
-int gy_upload(gy_ctx *c, const gy_particles *in) {
-  if (!c || !in || !in->pos || in->n <= 0) return fail(c, GY_ERR_INVALID, "upload needs n > 0 and positions");
-  InstHost &H = c->ih[0];
-  InstDesc &I = c->inst[0];
-  const int n = in->n;
-  const int nmat = (int)H.mats.size();
-  const double h = H.grid.cellsize, scale = (double)(1 << I.fbits) / h;
-  // multi-body contact (two velocity fields per node) is not on the single-body path
-  int obj0 = in->obj_id ? in->obj_id[0] : 0;
-  for (int i = 0; i < n; i++) {
-    const int m = in->material ? in->material[i] : 0;
-    if (m < 0 || m >= nmat) return fail(c, GY_ERR_INVALID, "particle %d: material %d out of range", i, m);
-    if (H.mats[m].is_rigid) return fail(c, GY_ERR_UNSUPPORTED, "rigid materials need two-body contact (not on this path)");
-    if (in->obj_id && in->obj_id[i] != obj0) return fail(c, GY_ERR_UNSUPPORTED, "more than one object ID needs two-body contact (not on this path)");
-    if (in->bc_type && (in->bc_type[i] < 0 || in->bc_type[i] > 127)) return fail(c, GY_ERR_INVALID, "particle %d: bad BC type", i);
+int gy_set_node_bc(gy_ctx *c, int32_t n_nodes, const int32_t *type, const double *val) {
+  if (!c || !type) return GY_ERR_INVALID;
+  if (n_nodes != c->nnodes) return fail(c, GY_ERR_INVALID, "expected %lld node BCs, got %d", c->nnodes, n_nodes);
+  return node_bc_impl(c, 0, n_nodes, type, val);
+}
+
+int gy_set_node_bc_inst(gy_ctx *c, int32_t inst, int32_t n_nodes, const int32_t *type, const double *val) {
+  if (!c || !type) return GY_ERR_INVALID;
+  if (inst < 0 || inst >= (int)c->inst.size()) return fail(c, GY_ERR_INVALID, "instance %d out of range", inst);
+  const InstDesc &I = c->inst[inst];
+  const long long nn = (long long)I.n[0] * I.n[1] * I.n[2];
+  if (n_nodes != nn) return fail(c, GY_ERR_INVALID, "instance %d: expected %lld node BCs, got %d", inst, nn, n_nodes);
+  return node_bc_impl(c, I.node_base, n_nodes, type, val);
+}
+
+static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
+  if (!c || !views) return GY_ERR_INVALID;
+  if (ninst != (int)c->inst.size())
+    return fail(c, GY_ERR_INVALID, "upload of %d instance(s) into a context of %d", ninst, (int)c->inst.size());
+  long long total = 0;
+  for (int k = 0; k < ninst; k++) {
+    const gy_particles *in = &views[k];
+    if (!in->pos || in->n <= 0) return fail(c, GY_ERR_INVALID, "instance %d: upload needs n > 0 and positions", k);
+    const InstHost &H = c->ih[k];
+    const int nmat = (int)H.mats.size();
+    // multi-body contact (two velocity fields per node) is not on the single-body path
+    const int obj0 = in->obj_id ? in->obj_id[0] : 0;
+    for (int i = 0; i < in->n; i++) {
+      const int m = in->material ? in->material[i] : 0;
+      if (m < 0 || m >= nmat) return fail(c, GY_ERR_INVALID, "instance %d particle %d: material %d out of range", k, i, m);
+      if (H.mats[m].is_rigid) return fail(c, GY_ERR_UNSUPPORTED, "rigid materials need two-body contact (not on this path)");
+      if (in->obj_id && in->obj_id[i] != obj0)
+        return fail(c, GY_ERR_UNSUPPORTED, "more than one object ID needs two-body contact (not on this path)");
+      if (in->bc_type && (in->bc_type[i] < 0 || in->bc_type[i] > 127))
+        return fail(c, GY_ERR_INVALID, "instance %d particle %d: bad BC type", k, i);
+    }
+    total += in->n;
   }
-  const long long cap = n;
+  if (total > 0x7fffffffLL - 1024) return fail(c, GY_ERR_UNSUPPORTED, "too many particles");
+  const long long cap = total;
   DevPtrs &d = c->d;
   int rc;
   for (int b = 0; b < 2; b++) {
@@ -427,77 +470,88 @@ int gy_upload(gy_ctx *c, const gy_particles *in) {
   if ((rc = dalloc(c, &d.pbc_val, 4 * (size_t)cap))) return rc;
   if ((rc = dalloc(c, &d.audit, 8 * ((size_t)(cap + 255) / 256 + 1)))) return rc;
   d.cap = cap;
-  d.np = n;
-  // host staging in the device layout
+  d.np = (int)total;
+  // host staging in the device layout; instance k owns slots [pstart, pstart + n)
   std::vector<uint32_t> buf((size_t)NF * cap, 0u);
-  auto F = [&](int f, int i) -> uint32_t & { return buf[(size_t)f * cap + i]; };
-  auto put = [&](int f, int i, double x) { float y = (float)x; std::memcpy(&F(f, i), &y, 4); };
-  H.carry_plastic.assign(nmat, 0);
-  H.carry_damage.assign(nmat, 0);
-  H.pos0.resize(3 * (size_t)n);
-  std::vector<float> pbc(4 * (size_t)n, 0.f);
-  const double psep3 = H.params.psep * H.params.psep * H.params.psep;
-  for (int i = 0; i < n; i++) {
-    for (int a = 0; a < 3; a++) {
-      const double x = in->pos[3 * i + a];
-      const double q = std::nearbyint(x * scale);
-      if (!(std::fabs(q) < 2.0e9)) return fail(c, GY_ERR_RANGE, "particle %d: position out of the fixed-point range", i);
-      F(F_QX + a, i) = (uint32_t)(int32_t)q;
-      H.pos0[3 * i + a] = in->pos0 ? in->pos0[3 * i + a] : x;
-      if (in->vel) put(F_VX + a, i, in->vel[3 * i + a]);
-    }
-    const int m = in->material ? in->material[i] : 0;
-    for (int k = 0; k < 6; k++) {
-      if (in->stress) put(F_S0 + k, i, in->stress[6 * i + k]);
-      if (in->strain) put(F_E0 + k, i, in->strain[6 * i + k]);
-      if (in->plastic_strain) {
-        put(F_P0 + k, i, in->plastic_strain[6 * i + k]);
-        if (in->plastic_strain[6 * i + k] != 0.0) H.carry_plastic[m] = 1;
-      }
-    }
-    if (in->acc_plastic_strain) {
-      put(F_EPB, i, in->acc_plastic_strain[i]);
-      if (in->acc_plastic_strain[i] != 0.0) H.carry_plastic[m] = 1;
-    }
-    // Solver::initialize (Solver.cpp:42-56) when density/mass are not given
-    const double rho = in->density ? in->density[i] : H.mats[m].density;
-    const double mass = in->mass ? in->mass[i] : (H.mats[m].is_rigid ? 1.0 : rho * psep3);
-    put(F_RHO, i, rho);
-    put(F_MASS, i, mass);
-    if (in->damage) { put(F_DMG, i, in->damage[i]); if (in->damage[i] != 0.0) H.carry_damage[m] = 1; }
-    if (in->smax) { put(F_SMAX, i, in->smax[i]); if (in->smax[i] != 0.0) H.carry_damage[m] = 1; }
-    const int bct = in->bc_type ? in->bc_type[i] : 0;
-    F(F_TAG, i) = make_tag(m, in->hold ? (in->hold[i] ? 1 : 0) : 0, bct, in->obj_id ? in->obj_id[i] : 0);
-    F(F_PID, i) = (uint32_t)i;
-    if (in->bc_val)
-      for (int k = 0; k < 4; k++) pbc[4 * (size_t)i + k] = (float)in->bc_val[4 * i + k];
-  }
-  for (int m = 0; m < nmat; m++) {
-    MatDev &M = c->mats[I.mat_base + m];
-    const MatDev base = make_mat(H.mats[m]);
-    M.carry_plastic = base.carry_plastic || H.carry_plastic[m];
-    M.carry_damage = base.carry_damage || H.carry_damage[m];
-    H.carry_plastic[m] = (uint8_t)M.carry_plastic;
-    H.carry_damage[m] = (uint8_t)M.carry_damage;
-  }
-  // model-set bits for the fused kernel (elastic 1, Newtonian 2, perfect 4, DP 8, GK 16,
-  // particle BCs 32); see k_g2p2g in gy_kernels.hip
+  auto F = [&](int f, long long s) -> uint32_t & { return buf[(size_t)f * cap + s]; };
+  auto put = [&](int f, long long s, double x) { float y = (float)x; std::memcpy(&F(f, s), &y, 4); };
+  std::vector<float> pbc(4 * (size_t)cap, 0.f);
   int kmask = 0;
-  std::vector<uint8_t> used(nmat, 0);
-  for (int i = 0; i < n; i++) used[in->material ? in->material[i] : 0] = 1;
-  for (int m = 0; m < nmat; m++) {
-    if (!used[m]) continue;
-    const gy_material &gm = H.mats[m];
-    kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : 8;
-    if (gm.is_damage && gm.damage_model == 1) kmask |= 16;
-    if (c->mats[I.mat_base + m].carry_damage && !(gm.is_damage && gm.damage_model == 1)) kmask |= 16;
+  long long pstart = 0;
+  for (int k = 0; k < ninst; k++) {
+    const gy_particles *in = &views[k];
+    InstHost &H = c->ih[k];
+    InstDesc &I = c->inst[k];
+    const int n = in->n;
+    const int nmat = (int)H.mats.size();
+    const double h = H.grid.cellsize, scale = (double)(1 << I.fbits) / h;
+    H.carry_plastic.assign(nmat, 0);
+    H.carry_damage.assign(nmat, 0);
+    H.pos0.resize(3 * (size_t)n);
+    const double psep3 = H.params.psep * H.params.psep * H.params.psep;
+    for (int i = 0; i < n; i++) {
+      const long long s_ = pstart + i;
+      for (int a = 0; a < 3; a++) {
+        const double x = in->pos[3 * i + a];
+        const double q = std::nearbyint(x * scale);
+        if (!(std::fabs(q) < 2.0e9)) return fail(c, GY_ERR_RANGE, "instance %d particle %d: position out of the fixed-point range", k, i);
+        F(F_QX + a, s_) = (uint32_t)(int32_t)q;
+        H.pos0[3 * i + a] = in->pos0 ? in->pos0[3 * i + a] : x;
+        if (in->vel) put(F_VX + a, s_, in->vel[3 * i + a]);
+      }
+      const int m = in->material ? in->material[i] : 0;
+      for (int kk = 0; kk < 6; kk++) {
+        if (in->stress) put(F_S0 + kk, s_, in->stress[6 * i + kk]);
+        if (in->strain) put(F_E0 + kk, s_, in->strain[6 * i + kk]);
+        if (in->plastic_strain) {
+          put(F_P0 + kk, s_, in->plastic_strain[6 * i + kk]);
+          if (in->plastic_strain[6 * i + kk] != 0.0) H.carry_plastic[m] = 1;
+        }
+      }
+      if (in->acc_plastic_strain) {
+        put(F_EPB, s_, in->acc_plastic_strain[i]);
+        if (in->acc_plastic_strain[i] != 0.0) H.carry_plastic[m] = 1;
+      }
+      // Solver::initialize (Solver.cpp:42-56) when density/mass are not given
+      const double rho = in->density ? in->density[i] : H.mats[m].density;
+      const double mass = in->mass ? in->mass[i] : (H.mats[m].is_rigid ? 1.0 : rho * psep3);
+      put(F_RHO, s_, rho);
+      put(F_MASS, s_, mass);
+      if (in->damage) { put(F_DMG, s_, in->damage[i]); if (in->damage[i] != 0.0) H.carry_damage[m] = 1; }
+      if (in->smax) { put(F_SMAX, s_, in->smax[i]); if (in->smax[i] != 0.0) H.carry_damage[m] = 1; }
+      const int bct = in->bc_type ? in->bc_type[i] : 0;
+      F(F_TAG, s_) = make_tag(m, in->hold ? (in->hold[i] ? 1 : 0) : 0, bct, in->obj_id ? in->obj_id[i] : 0);
+      F(F_PID, s_) = (uint32_t)s_;
+      if (in->bc_val)
+        for (int kk = 0; kk < 4; kk++) pbc[4 * (size_t)s_ + kk] = (float)in->bc_val[4 * i + kk];
+    }
+    for (int m = 0; m < nmat; m++) {
+      MatDev &M = c->mats[I.mat_base + m];
+      const MatDev base = make_mat(H.mats[m]);
+      M.carry_plastic = base.carry_plastic || H.carry_plastic[m];
+      M.carry_damage = base.carry_damage || H.carry_damage[m];
+      H.carry_plastic[m] = (uint8_t)M.carry_plastic;
+      H.carry_damage[m] = (uint8_t)M.carry_damage;
+    }
+    // model-set bits for the fused kernel (elastic 1, Newtonian 2, perfect 4, DP 8, GK 16,
+    // particle BCs 32); see k_g2p2g in gy_kernels.hip
+    std::vector<uint8_t> used(nmat, 0);
+    for (int i = 0; i < n; i++) used[in->material ? in->material[i] : 0] = 1;
+    for (int m = 0; m < nmat; m++) {
+      if (!used[m]) continue;
+      const gy_material &gm = H.mats[m];
+      kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : 8;
+      if (gm.is_damage && gm.damage_model == 1) kmask |= 16;
+      if (c->mats[I.mat_base + m].carry_damage && !(gm.is_damage && gm.damage_model == 1)) kmask |= 16;
+    }
+    for (int i = 0; in->bc_type && i < n; i++)
+      if (in->bc_type[i]) { kmask |= 32; break; }
+    H.nparticles = n;
+    I.pstart = (int)pstart;
+    I.pcount = n;
+    pstart += n;
   }
-  for (int i = 0; in->bc_type && i < n; i++)
-    if (in->bc_type[i]) { kmask |= 32; break; }
   d.kmask = kmask;
-  H.nparticles = n;
-  I.pstart = 0;
-  I.pcount = n;
   HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(d.pbc_val, pbc.data(), pbc.size() * 4, hipMemcpyHostToDevice, c->stream));
   if ((rc = push_inst(c))) return rc;
@@ -509,6 +563,13 @@ int gy_upload(gy_ctx *c, const gy_particles *in) {
   drop_graph(c);
   return GY_OK;
 }
+
+int gy_upload(gy_ctx *c, const gy_particles *in) {
+  if (c && c->inst.size() != 1) return fail(c, GY_ERR_INVALID, "batched context: use gy_upload_batch");
+  return upload_impl(c, 1, in);
+}
+
+int gy_upload_batch(gy_ctx *c, int32_t ninst, const gy_particles *views) { return upload_impl(c, ninst, views); }
 
 int gy_set_dt(gy_ctx *c, double dt) {
   if (!c || !(dt > 0)) return GY_ERR_INVALID;
@@ -526,6 +587,10 @@ int gy_set_dt(gy_ctx *c, double dt) {
 }
 
 double gy_get_dt(const gy_ctx *c) { return c && !c->ih.empty() ? c->ih[0].dt : 0.0; }
+
+double gy_get_dt_inst(const gy_ctx *c, int32_t inst) {
+  return (c && inst >= 0 && inst < (int)c->ih.size()) ? c->ih[inst].dt : 0.0;
+}
 
 static int iterate_impl(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
   if (!c) return GY_ERR_INVALID;
@@ -564,27 +629,31 @@ int gy_synchronize(gy_ctx *c) {
   return check_status(c);
 }
 
-int gy_download(gy_ctx *c, gy_particles *o) {
+static int download_impl(gy_ctx *c, int inst, gy_particles *o) {
   if (!c || !o) return GY_ERR_INVALID;
   if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  if (inst < 0 || inst >= (int)c->inst.size()) return fail(c, GY_ERR_INVALID, "instance %d out of range", inst);
   int rc = gy_synchronize(c);
   if (rc && rc != GY_ERR_RANGE) return rc;
   Ctrl h{};
   HIPCHECK(c, hipMemcpy(&h, c->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
   const long long cap = c->d.cap;
-  const int n = c->d.np;
-  if (o->n != n) return fail(c, GY_ERR_INVALID, "download view has n=%d, context has %d", o->n, n);
-  std::vector<uint32_t> buf((size_t)NF * cap);
-  HIPCHECK(c, hipMemcpy(buf.data(), c->d.pbuf[h.parity], buf.size() * 4, hipMemcpyDeviceToHost));
-  const InstHost &H = c->ih[0];
-  const InstDesc &I = c->inst[0];
+  const InstHost &H = c->ih[inst];
+  const InstDesc &I = c->inst[inst];
+  const int n = I.pcount;
+  if (o->n != n) return fail(c, GY_ERR_INVALID, "download view has n=%d, instance %d has %d", o->n, inst, n);
+  // the instance's particles occupy slots [pstart, pstart + pcount) in every binning
+  std::vector<uint32_t> buf((size_t)NF * n);
+  for (int f = 0; f < NF; f++)
+    HIPCHECK(c, hipMemcpy(buf.data() + (size_t)f * n, c->d.pbuf[h.parity] + (size_t)f * cap + I.pstart,
+                          (size_t)n * 4, hipMemcpyDeviceToHost));
   const double h_ = H.grid.cellsize, inv = h_ / (double)(1 << I.fbits);
-  auto U = [&](int f, int s) { return buf[(size_t)f * cap + s]; };
+  auto U = [&](int f, int s) { return buf[(size_t)f * n + s]; };
   auto Fv = [&](int f, int s) { float y; uint32_t u = U(f, s); std::memcpy(&y, &u, 4); return (double)y; };
   const int Cx = H.n[0] - 1, Cy = H.n[1] - 1;
   for (int s = 0; s < n; s++) {
-    const int i = (int)U(F_PID, s);
-    if (i < 0 || i >= n) return fail(c, GY_ERR_STATE, "corrupt particle id at slot %d", s);
+    const int i = (int)U(F_PID, s) - I.pstart;
+    if (i < 0 || i >= n) return fail(c, GY_ERR_STATE, "corrupt particle id at slot %d", I.pstart + s);
     const uint32_t tag = U(F_TAG, s);
     const int m = tag_mat(tag);
     double x[3];
@@ -618,6 +687,31 @@ int gy_download(gy_ctx *c, gy_particles *o) {
     }
   }
   return rc;
+}
+
+int gy_download(gy_ctx *c, gy_particles *o) {
+  if (c && c->inst.size() != 1) return fail(c, GY_ERR_INVALID, "batched context: use gy_download_inst");
+  return download_impl(c, 0, o);
+}
+
+int gy_download_inst(gy_ctx *c, int32_t inst, gy_particles *o) { return download_impl(c, inst, o); }
+
+int gy_download_nodes(gy_ctx *c, int32_t n_nodes, double *mass, double *vel, double *force) {
+  if (!c) return GY_ERR_INVALID;
+  if (n_nodes != c->nnodes) return fail(c, GY_ERR_INVALID, "expected %lld nodes, got %d", c->nnodes, n_nodes);
+  int rc = gy_synchronize(c);
+  if (rc && rc != GY_ERR_RANGE) return rc;
+  std::vector<float> rec((size_t)n_nodes * NODE_W);
+  HIPCHECK(c, hipMemcpy(rec.data(), c->d.vel, rec.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_nodes; i++) {
+    const float *r = &rec[(size_t)i * NODE_W];  // vx vy vz ax ay az m -
+    if (mass) mass[i] = r[6];
+    for (int a = 0; a < 3; a++) {
+      if (vel) vel[3 * i + a] = r[a];
+      if (force) force[3 * i + a] = (double)r[6] * (double)r[3 + a];
+    }
+  }
+  return GY_OK;
 }
 
 int gy_audit(gy_ctx *c, gy_audit_out *out) {
@@ -723,6 +817,70 @@ int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *cou
   return rc;
 }
 
+int gy_autotune(gy_ctx *c, int32_t trial_steps, int64_t *best_wgs, double *best_ms) {
+  if (!c) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  if (trial_steps < 1 || trial_steps > 1000) return fail(c, GY_ERR_INVALID, "trial_steps must be in [1, 1000]");
+  int rc = gy_synchronize(c);
+  if (rc) return rc;
+  DevPtrs &d = c->d;
+  Ctrl h{};
+  HIPCHECK(c, hipMemcpy(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  // snapshot of the particle state and per-instance step times; trials run on the live
+  // buffers and the snapshot is put back afterwards (re-binned by the next step)
+  const size_t words = (size_t)NF * d.cap;
+  uint32_t *snap = nullptr;
+  HIPCHECK(c, hipMalloc((void **)&snap, words * 4));
+  std::vector<double> tl(c->inst.size());
+  HIPCHECK(c, hipMemcpy(snap, d.pbuf[h.parity], words * 4, hipMemcpyDeviceToDevice));
+  HIPCHECK(c, hipMemcpy(tl.data(), d.t_last, tl.size() * sizeof(double), hipMemcpyDeviceToHost));
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
+  const int per_cu[] = {1, 2, 3, 4, 6};
+  const bool saved_prof = c->profile;
+  c->profile = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int best = d.persist_wgs;
+  double best_t = 1e300;
+  auto restore = [&]() -> int {
+    HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], snap, words * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d.t_last, tl.data(), tl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_set_ctrl(d, h.step, 0.0, 1, c->stream);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->acc_valid = false;
+    return GY_OK;
+  };
+  for (int k : per_cu) {
+    const int wgs = 8 * ((cus * k + 7) / 8);
+    d.persist_wgs = wgs;
+    drop_graph(c);
+    if ((rc = restore())) break;
+    if ((rc = iterate_impl(c, 0.0, h.step, 1))) break;  // binning + first step (untimed)
+    (void)hipEventRecord(e0, c->stream);
+    if ((rc = iterate_impl(c, 0.0, h.step + 1, trial_steps))) break;
+    (void)hipEventRecord(e1, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = fail(c, GY_ERR_HIP, "autotune trial failed"); break; }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / trial_steps;
+    if (per < best_t) { best_t = per; best = wgs; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->profile = saved_prof;
+  d.persist_wgs = best;
+  drop_graph(c);
+  const int rc2 = restore();
+  (void)hipFree(snap);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (best_wgs) *best_wgs = best;
+  if (best_ms) *best_ms = best_t;
+  return GY_OK;
+}
+
 int gy_set_option(gy_ctx *c, int32_t option, int64_t value) {
   if (!c) return GY_ERR_INVALID;
   switch (option) {
@@ -769,7 +927,7 @@ void gy_destroy(gy_ctx *c) {
   dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.bc_type); dfree(d.bc_val);
   dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
   dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
-  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.partial);
+  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.pcore); dfree(d.partial);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -26,7 +26,10 @@ constexpr int G2P_R = TILE + 2;            // staged VEL region edge (nodes)
 constexpr int P2G_R = TILE + 4;            // LDS accumulator region edge (nodes), halo -1/+1
 constexpr int NODE_W = 8;                  // floats per node record
 constexpr int P2G_NODES = P2G_R * P2G_R * P2G_R;  // nodes of a tile's P2G region
-constexpr int PARTIAL_W = 7 * P2G_NODES;   // floats of one tile's partial sums
+constexpr int PARTIAL_W = 7 * P2G_NODES;   // floats of one tile's full-region partial sums
+constexpr int CORE_R = TILE + 2;           // edge of the region a tile's own cells reach
+constexpr int CORE_NODES = CORE_R * CORE_R * CORE_R;
+constexpr int CORE_W = 7 * CORE_NODES;     // floats of one tile's core partial sums
 
 enum PField : int {
   F_QX = 0, F_QY, F_QZ,                     // fixed-point position (int32 bits)
@@ -123,8 +126,12 @@ struct DevPtrs {
   int *blocksum;         // scan partials (particle counts per scan block)
   int *blocktiles;       // scan partials (active tiles per scan block)
   int *active;           // active tile list, ascending tile id
-  int *tile_epoch;       // per tile: epoch of its last written partials
-  float *partial;        // per tile: P2G sums of its LDS region, [7][P2G_R^3] floats
+  int *tile_epoch;       // per tile: epoch of its last written core partials
+  int *halo_epoch;       // per tile: epoch of its last written halo-shell partials
+  float *pcore;          // per tile: P2G sums of the core [o, o+TILE+2)^3, [7][CORE_R^3]
+  float *partial;        // per tile: P2G sums of the LDS region [o-1, o+TILE+3)^3,
+                         // [7][P2G_R^3]; only the shell outside the core is read, and
+                         // only when halo_epoch is current
   float *acc;            // NODE_W per global node: P2G of particles outside their tile's region
   float *vel;            // NODE_W per global node
   uint8_t *brick_flag;   // per global canonical brick

@@ -536,8 +536,11 @@ __global__ __launch_bounds__(WG) void k_stable_order(DevPtrs d, int *order) {
 
 // ---------------------------------------------------------------------------------------
 // Sum of the tile partials covering actual node (a0, a1, a2): a node n lies in the P2G
-// region [o-1, o+TILE+3) of the tiles with o = TILE*(tt-1) and TILE*tt, tt = (n+1)/TILE.
-// Only tiles written this step (tile_epoch == epoch) count. Fixed order: deterministic.
+// region [o-1, o+TILE+3) of the tiles with o = TILE*(tt-1) and TILE*tt, tt = (n+1)/TILE,
+// at region coordinate l = n - o + 1. Coordinates 1..TILE+2 on every axis are the tile's
+// core (read when the tile wrote partials this step, tile_epoch == epoch); the rest is
+// the halo shell, which only particles that changed cell reach (read when halo_epoch ==
+// epoch). Fixed order: deterministic.
 __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc &I, int epoch, int a0, int a1,
                                                 int a2, float acc7[7]) {
   const int t0 = (a0 + 1) / TILE, t1 = (a1 + 1) / TILE, t2 = (a2 + 1) / TILE;
@@ -545,20 +548,32 @@ __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc
   for (int dz = 0; dz < 2; dz++) {
     const int tz = t2 - 1 + dz;
     if (tz < 0 || tz >= I.nt[2]) continue;
+    const int lz = a2 - TILE * tz + 1;
 #pragma unroll
     for (int dy = 0; dy < 2; dy++) {
       const int ty = t1 - 1 + dy;
       if (ty < 0 || ty >= I.nt[1]) continue;
+      const int ly = a1 - TILE * ty + 1;
 #pragma unroll
       for (int dx = 0; dx < 2; dx++) {
         const int tx = t0 - 1 + dx;
         if (tx < 0 || tx >= I.nt[0]) continue;
+        const int lx = a0 - TILE * tx + 1;
         const int t = I.tile_base + tx + I.nt[0] * (ty + I.nt[1] * tz);
-        if (d.tile_epoch[t] != epoch) continue;
-        const int l = (a0 - TILE * tx + 1) + P2G_R * ((a1 - TILE * ty + 1) + P2G_R * (a2 - TILE * tz + 1));
-        const float *pp = d.partial + (long long)t * PARTIAL_W + l;
+        const bool core = lx >= 1 && lx <= CORE_R && ly >= 1 && ly <= CORE_R && lz >= 1 && lz <= CORE_R;
+        const float *pp;
+        int stride;
+        if (core) {
+          if (d.tile_epoch[t] != epoch) continue;
+          pp = d.pcore + (long long)t * CORE_W + (lx - 1) + CORE_R * ((ly - 1) + CORE_R * (lz - 1));
+          stride = CORE_NODES;
+        } else {
+          if (d.halo_epoch[t] != epoch) continue;
+          pp = d.partial + (long long)t * PARTIAL_W + lx + P2G_R * (ly + P2G_R * lz);
+          stride = P2G_NODES;
+        }
 #pragma unroll
-        for (int f = 0; f < 7; f++) acc7[f] += pp[f * P2G_NODES];
+        for (int f = 0; f < 7; f++) acc7[f] += pp[f * stride];
       }
     }
   }
@@ -717,6 +732,9 @@ constexpr int CHUNK = WG;
 constexpr int ACC_YS = P2G_R;
 constexpr int ACC_ZS = P2G_R * P2G_R + 4;
 constexpr int ACC_N = ACC_ZS * P2G_R;
+// Phase B scatters particles one per lane (false) or two of one cell per lane (true:
+// half the LDS atomics, twice the weights per lane; measured slower on MI355X).
+constexpr bool P2G_PAIRS = false;
 // P2G record of one particle, handed from the update phase to the scatter phase in LDS:
 // fixed-point position (3), m, m*v (3), -V*sigma (6).
 constexpr int NREC = 13;
@@ -943,49 +961,93 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   return knew;
 }
 
-// Phase B for one record: GIMP weights of its position and the 27-node scatter of m, m*v
-// and -V*sigma.gradN into the tile's FP64 LDS region (Extrapolation.cpp:166-286).
-__device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
-                                                 const float (*s_rec)[CHUNK], int j, double (*s_acc)[ACC_N]) {
-  int q[3], b[3];
-  bool bad = false;
+// Phase B: GIMP weights of one record's position (Extrapolation.cpp:556-709).
+struct ScatterRec {
+  int b[3];
   Axis W[3];
+  P2GTerms T;
+};
+__device__ __forceinline__ void load_rec(const InstDesc &I, const float (*s_rec)[CHUNK], int j, ScatterRec &R) {
+  bool bad = false;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    q[a] = __float_as_int(s_rec[a][j]);
-    b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
-    W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+    const int q = __float_as_int(s_rec[a][j]);
+    R.b[a] = stencil_base(q, I.lpq, I.fbits, I.n[a] - 3, bad);
+    R.W[a] = gimp3((float)(q - (R.b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
   }
-  P2GTerms T;
-  T.m = s_rec[3][j];
+  R.T.m = s_rec[3][j];
 #pragma unroll
-  for (int a = 0; a < 3; a++) T.mv[a] = s_rec[4 + a][j];
+  for (int a = 0; a < 3; a++) R.T.mv[a] = s_rec[4 + a][j];
 #pragma unroll
-  for (int a = 0; a < 6; a++) T.vs[a] = s_rec[7 + a][j];
-  const int r0 = b[0] - o[0] + 1, r1 = b[1] - o[1] + 1, r2 = b[2] - o[2] + 1;
-  const bool inside = (r0 >= 0 && r1 >= 0 && r2 >= 0 && r0 + 2 < P2G_R && r1 + 2 < P2G_R && r2 + 2 < P2G_R);
-  if (inside) {
+  for (int a = 0; a < 6; a++) R.T.vs[a] = s_rec[7 + a][j];
+}
+
+__device__ __forceinline__ bool region_offset(const ScatterRec &R, const int o[3], int r[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) r[a] = R.b[a] - o[a] + 1;
+  return r[0] >= 0 && r[1] >= 0 && r[2] >= 0 && r[0] + 2 < P2G_R && r[1] + 2 < P2G_R && r[2] + 2 < P2G_R;
+}
+
+// 27-node scatter of m, m*v and -V*sigma.gradN (Extrapolation.cpp:166-286) of one or two
+// particles with the same stencil base into the tile's FP64 LDS region: the two
+// particles' FP32 terms of a node are added before the one FP64 atomic per field.
+template <bool TWO>
+__device__ __forceinline__ void scatter_lds(const ScatterRec &P, const ScatterRec &Q, const int r[3],
+                                            double (*s_acc)[ACC_N]) {
 #pragma unroll 1
-    for (int kk = 0; kk < 3; kk++) {
-      const float wz = sel3(kk, W[2].w[0], W[2].w[1], W[2].w[2]), gz = sel3(kk, W[2].g[0], W[2].g[1], W[2].g[2]);
+  for (int kk = 0; kk < 3; kk++) {
+    const float wz = sel3(kk, P.W[2].w[0], P.W[2].w[1], P.W[2].w[2]);
+    const float gz = sel3(kk, P.W[2].g[0], P.W[2].g[1], P.W[2].g[2]);
+    const float wz2 = TWO ? sel3(kk, Q.W[2].w[0], Q.W[2].w[1], Q.W[2].w[2]) : 0.0f;
+    const float gz2 = TWO ? sel3(kk, Q.W[2].g[0], Q.W[2].g[1], Q.W[2].g[2]) : 0.0f;
 #pragma unroll
-      for (int jj = 0; jj < 3; jj++) {
-        const float Aw = W[1].w[jj] * wz;
-        const float By = W[1].g[jj] * wz;
-        const float Bz = W[1].w[jj] * gz;
-        const int row = r0 + ACC_YS * (r1 + jj) + ACC_ZS * (r2 + kk);
+    for (int jj = 0; jj < 3; jj++) {
+      const float Aw = P.W[1].w[jj] * wz, By = P.W[1].g[jj] * wz, Bz = P.W[1].w[jj] * gz;
+      const float Aw2 = TWO ? Q.W[1].w[jj] * wz2 : 0.0f, By2 = TWO ? Q.W[1].g[jj] * wz2 : 0.0f,
+                  Bz2 = TWO ? Q.W[1].w[jj] * gz2 : 0.0f;
+      const int row = r[0] + ACC_YS * (r[1] + jj) + ACC_ZS * (r[2] + kk);
 #pragma unroll
-        for (int ii2 = 0; ii2 < 3; ii2++) {
-          float tt[7];
-          node_terms(T, W[0].w[ii2] * Aw, W[0].g[ii2] * Aw, W[0].w[ii2] * By, W[0].w[ii2] * Bz, tt);
+      for (int ii2 = 0; ii2 < 3; ii2++) {
+        float tt[7];
+        node_terms(P.T, P.W[0].w[ii2] * Aw, P.W[0].g[ii2] * Aw, P.W[0].w[ii2] * By, P.W[0].w[ii2] * Bz, tt);
+        if (TWO) {
+          float t2[7];
+          node_terms(Q.T, Q.W[0].w[ii2] * Aw2, Q.W[0].g[ii2] * Aw2, Q.W[0].w[ii2] * By2, Q.W[0].w[ii2] * Bz2, t2);
 #pragma unroll
-          for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], (double)tt[f]);
+          for (int f = 0; f < 7; f++) tt[f] += t2[f];
         }
+#pragma unroll
+        for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], (double)tt[f]);
       }
     }
-  } else {
-    p2g_global(d.acc, d.brick_flag, &d.inst[inst], b[0], b[1], b[2], W[0], W[1], W[2], T);
   }
+}
+
+__device__ __forceinline__ void scatter_one(const DevPtrs &d, int inst, const ScatterRec &P, const int o[3],
+                                            double (*s_acc)[ACC_N]) {
+  int r[3];
+  if (region_offset(P, o, r)) scatter_lds<false>(P, P, r, s_acc);
+  else p2g_global(d.acc, d.brick_flag, &d.inst[inst], P.b[0], P.b[1], P.b[2], P.W[0], P.W[1], P.W[2], P.T);
+}
+
+// Phase B for one list item: record j alone, or records j and j+1 (two particles of one
+// cell) when `two`.
+__device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
+                                                 const float (*s_rec)[CHUNK], int j, bool two,
+                                                 double (*s_acc)[ACC_N]) {
+  ScatterRec P, Q;
+  load_rec(I, s_rec, j, P);
+  if (P2G_PAIRS && two) {
+    load_rec(I, s_rec, j + 1, Q);
+    int r[3];
+    const bool same = P.b[0] == Q.b[0] && P.b[1] == Q.b[1] && P.b[2] == Q.b[2];
+    if (same && region_offset(P, o, r)) {
+      scatter_lds<true>(P, Q, r, s_acc);
+      return;
+    }
+    scatter_one(d, inst, Q, o, s_acc);  // the pair's stencils differ (a particle changed cell)
+  }
+  scatter_one(d, inst, P, o, s_acc);
 }
 
 // The fused particle kernel, persistent over the active tile list. UPDATE = false: P2G of
@@ -997,9 +1059,10 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 //     node velocities, update, constitutive/damage models, store at the entry's sorted
 //     position, new key with one histogram atomic per run of equal keys, and the P2G
 //     record into LDS;
-//  B. lane = entry in rank-major order (the r-th particles of every cell first), so the
-//     lanes of one LDS atomic belong to different cells and hit different nodes: 27-node
-//     scatter into the FP64 LDS region [o-1, o+TILE+3)^3.
+//  B. lane = pair of entries of one cell, pairs in rank-major order (the r-th pairs of
+//     every cell first), so the lanes of one LDS atomic belong to different cells and hit
+//     different nodes: 27-node scatter of both particles into the FP64 LDS region
+//     [o-1, o+TILE+3)^3, one atomic per node and field for the pair.
 // ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32 LDS atomic adds. After the last
 // chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
 // is stamped; the grid kernel gathers the partials of the tiles covering each node in a
@@ -1013,7 +1076,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ unsigned short s_list[CHUNK];
   __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
-  __shared__ int s_tile;
+  __shared__ int s_tile, s_nitems;
   const int tid = threadIdx.x;
   const int grp = blockIdx.x & 7;  // dispatch XCD of this workgroup (round-robin)
   const int nact = d.ctrl->active_tiles;
@@ -1027,7 +1090,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       s_tile = (jt < hi) ? d.active[jt] : -1;
     }
     __syncthreads();
-    const int t = s_tile;
+    // workgroup-uniform values read from LDS: keep them (and all that derives from
+    // them, the instance description included) in scalar registers
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
     if (t < 0) break;
     const int k0 = t * TILE3;
     const int ii = d.tile_inst[t];
@@ -1059,23 +1124,30 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __syncthreads();
 
     const double tl = UPDATE ? d.t_last[ii] : 0.0;
-    const int p0 = s_off[0], p1 = s_off[TILE3];
+    const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
     for (int c0 = p0; c0 < p1; c0 += CHUNK) {
       const int c1 = min(c0 + CHUNK, p1), nc = c1 - c0;
-      // rank-major order of this chunk's entries (wave 0, lane = cell of the tile)
+      // rank-major order of this chunk's entries, or of pairs of one cell's entries (wave
+      // 0, lane = cell of the tile): item = first entry | (second entry present) << 15
       if (tid < TILE3) {
         const int elo = max(s_off[tid], c0), ehi = min(s_off[tid + 1], c1);
         const int n = max(ehi - elo, 0);
-        int maxn = n;
+        const int np2 = P2G_PAIRS ? (n + 1) >> 1 : n;
+        int maxn = np2;
 #pragma unroll
         for (int w = 32; w > 0; w >>= 1) maxn = max(maxn, __shfl_xor(maxn, w, 64));
         const unsigned long long below = (1ull << tid) - 1ull;
         int S = 0;
         for (int r = 0; r < maxn; r++) {
-          const unsigned long long m = __ballot(n > r);
-          if (n > r) s_list[S + __popcll(m & below)] = (unsigned short)(elo - c0 + r);
+          const unsigned long long m = __ballot(np2 > r);
+          if (np2 > r) {
+            const int e = elo - c0 + (P2G_PAIRS ? 2 * r : r);
+            s_list[S + __popcll(m & below)] =
+                (unsigned short)(e | ((P2G_PAIRS && 2 * r + 1 < n) ? 0x8000 : 0));
+          }
           S += __popcll(m);
         }
+        if (tid == 0) s_nitems = S;
       }
       // phase A
       const int i = c0 + tid;
@@ -1087,23 +1159,44 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       }
       __syncthreads();
       // phase B
-      if (tid < nc) particle_scatter(d, I, o, ii, s_rec, s_list[tid], s_acc);
+      if (tid < __builtin_amdgcn_readfirstlane(s_nitems)) {
+        const int it = s_list[tid];
+        particle_scatter(d, I, o, ii, s_rec, it & 0x7fff, (it & 0x8000) != 0, s_acc);
+      }
       __syncthreads();
     }
 
-    // ---- flush: the region's sums as FP32 partials (plain stores), epoch, brick flags ----
-    float *pt = d.partial + (long long)t * PARTIAL_W;
+    // ---- flush: core sums as FP32 partials (plain stores), the halo shell only when a
+    // particle reached it, brick flags, epochs ----
+    float *pc = d.pcore + (long long)t * CORE_W;
+    bool halo = false;
     for (int n = tid; n < P2G_NODES; n += WG) {
       const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
       const int a = x + ACC_YS * y + ACC_ZS * z;
+      const bool shell = x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R;
+      if (!shell) {
+        const int c = (x - 1) + CORE_R * ((y - 1) + CORE_R * (z - 1));
 #pragma unroll
-      for (int f = 0; f < 7; f++) pt[f * P2G_NODES + n] = (float)s_acc[f][a];
+        for (int f = 0; f < 7; f++) pc[f * CORE_NODES + c] = (float)s_acc[f][a];
+      }
       if (s_acc[0][a] == 0.0) continue;
+      halo = halo || shell;
       const int g[3] = {o[0] - 1 + x, o[1] - 1 + y, o[2] - 1 + z};
       if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
       int c[3];
       (void)canon_index(I, g, c);
       flag_brick(d, I, c);
+    }
+    if (__syncthreads_or(halo)) {
+      float *pt = d.partial + (long long)t * PARTIAL_W;
+      for (int n = tid; n < P2G_NODES; n += WG) {
+        const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
+        if (!(x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R)) continue;
+        const int a = x + ACC_YS * y + ACC_ZS * z;
+#pragma unroll
+        for (int f = 0; f < 7; f++) pt[f * P2G_NODES + n] = (float)s_acc[f][a];
+      }
+      if (tid == 0) d.halo_epoch[t] = epoch_w;
     }
     if (tid == 0) d.tile_epoch[t] = epoch_w;
     __syncthreads();

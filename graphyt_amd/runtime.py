@@ -31,9 +31,18 @@ SIGNATURES = {
     "gy_iterate_n": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32]),
     "gy_synchronize": (C.c_int, [_P]),
     "gy_download": (C.c_int, [_P, C.POINTER(GyParticles)]),
+    "gy_download_nodes": (C.c_int, [_P, C.c_int32, _DP, _DP, _DP]),
     "gy_audit": (C.c_int, [_P, C.POINTER(GyAudit)]),
     "gy_debug_binning": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I64P, _I64P]),
     "gy_counting_sort": (C.c_int, [_I32P, C.c_int64, C.c_int64, _I32P, _I32P, _I32P]),
+    "gy_create_batch": (C.c_int, [_PP, C.c_int32, C.POINTER(GyGrid), C.POINTER(GyMaterial), _I32P,
+                                  C.POINTER(GyParams)]),
+    "gy_instance_count": (C.c_int, [_P]),
+    "gy_set_node_bc_inst": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP]),
+    "gy_upload_batch": (C.c_int, [_P, C.c_int32, C.POINTER(GyParticles)]),
+    "gy_download_inst": (C.c_int, [_P, C.c_int32, C.POINTER(GyParticles)]),
+    "gy_get_dt_inst": (C.c_double, [_P, C.c_int32]),
+    "gy_autotune": (C.c_int, [_P, C.c_int32, _I64P, _DP]),
     "gy_set_option": (C.c_int, [_P, C.c_int32, C.c_int64]),
     "gy_kernel_count": (C.c_int, []),
     "gy_kernel_name": (C.c_char_p, [C.c_int32]),
@@ -128,6 +137,17 @@ class Context:
         self._check(self.lib.gy_download(self.h, C.byref(pa.view())))
         return pa.arrays
 
+    def download_nodes(self) -> dict:
+        """Node mass, centre-of-mass velocity and force (m * a after BCs) of the last step."""
+        nn = self.scene.num_nodes
+        m, v, f = np.zeros(nn), np.zeros(3 * nn), np.zeros(3 * nn)
+        self._check(self.lib.gy_download_nodes(self.h, nn, m.ctypes.data_as(_DP), v.ctypes.data_as(_DP),
+                                               f.ctypes.data_as(_DP)))
+        return {"mass": m, "vel": v.reshape(-1, 3), "force": f.reshape(-1, 3)}
+
+    def autotune(self, trial_steps: int = 20):
+        return autotune(self, trial_steps)
+
     def audit(self) -> dict:
         a = GyAudit()
         self._check(self.lib.gy_audit(self.h, C.byref(a)))
@@ -173,6 +193,72 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class BatchContext:
+    """Many independent scenes advanced together (gy_create_batch / gy_upload_batch):
+    every step call advances all of them in the same launches."""
+
+    def __init__(self, scenes, graph: bool = True):
+        self.lib = load()
+        self.scenes = list(scenes)
+        k = len(self.scenes)
+        grids = (GyGrid * k)(*[sc.grid() for sc in self.scenes])
+        params = (GyParams * k)(*[sc.params() for sc in self.scenes])
+        mats = [m for sc in self.scenes for m in sc.mats]
+        self._mats = material_array(mats)
+        nmat = np.array([len(sc.mats) for sc in self.scenes], np.int32)
+        h = C.c_void_p()
+        rc = self.lib.gy_create_batch(C.byref(h), k, grids, self._mats, nmat.ctypes.data_as(_I32P), params)
+        self.h = h
+        self._check(rc)
+        for i, sc in enumerate(self.scenes):
+            if sc.node_bc_type is not None:
+                t = _i32(sc.node_bc_type)
+                v = np.ascontiguousarray(sc.node_bc_val, np.float64)
+                self._check(self.lib.gy_set_node_bc_inst(self.h, i, len(t), t.ctypes.data_as(_I32P),
+                                                         v.ctypes.data_as(_DP)))
+        self._views = (GyParticles * k)(*[sc.particles.view() for sc in self.scenes])
+        self._check(self.lib.gy_upload_batch(self.h, k, self._views))
+        self.set_option(OPT_USE_GRAPH, int(graph))
+
+    _check = Context._check
+    set_option = Context.set_option
+    run = Context.run
+    iterate = Context.iterate
+    synchronize = Context.synchronize
+    audit = Context.audit
+    kernel_stats = Context.kernel_stats
+    reset_kernel_stats = Context.reset_kernel_stats
+    close = Context.close
+    __del__ = Context.__del__
+
+    @property
+    def n(self) -> int:
+        return sum(sc.n for sc in self.scenes)
+
+    @property
+    def dt(self) -> float:
+        return self.lib.gy_get_dt(self.h)
+
+    def dt_of(self, i: int) -> float:
+        return self.lib.gy_get_dt_inst(self.h, i)
+
+    def download(self, i: int, fields=("pos", "vel", "stress", "plastic_strain", "acc_plastic_strain", "density")):
+        pa = ParticleArrays.empty(self.scenes[i].n, fields)
+        self._check(self.lib.gy_download_inst(self.h, i, C.byref(pa.view())))
+        return pa.arrays
+
+    def autotune(self, trial_steps: int = 20):
+        return autotune(self, trial_steps)
+
+
+def autotune(ctx, trial_steps: int = 20) -> dict:
+    """gy_autotune: pick the persistent-kernel workgroup count by timing trial steps; the
+    particle state is restored afterwards."""
+    wgs, ms = C.c_int64(), C.c_double()
+    ctx._check(ctx.lib.gy_autotune(ctx.h, trial_steps, C.byref(wgs), C.byref(ms)))
+    return {"persist_wgs": wgs.value, "ms_per_step": ms.value}
 
 
 def counting_sort(keys, nkeys: int):

@@ -176,10 +176,19 @@ def dp_column_mid(steps=1000, **kw) -> Scene:
                      steps=steps, **kw)
 
 
+def exact_length(n_nodes: int, h: float) -> float:
+    """A domain length L near (n_nodes - 1) h whose FP64 quotient L/h truncates to
+    n_nodes - 1, so Nx = int(1 + L/h) (Nodes.cpp:71-73) gives n_nodes nodes."""
+    L = (n_nodes - 1) * h
+    while int(1 + L / h) < n_nodes:
+        L = float(np.nextafter(L, np.inf))
+    return L
+
+
 def impact_block(h=0.01, n_nodes=24, count=20000, K=1e11, G=4e10, sy=5e8, v=100.0,
                  model=3, seed=SEED) -> Scene:
     """Perfect-plastic steel block moving at -v along x toward a fixed grid_vx wall."""
-    L = (n_nodes - 1) * h
+    L = exact_length(n_nodes, h)
     psep = h / 2
     side = max(1, int(round(count ** (1.0 / 3.0))))
     size = side * psep
@@ -196,3 +205,31 @@ def impact_block(h=0.01, n_nodes=24, count=20000, K=1e11, G=4e10, sy=5e8, v=100.
     s.node_bc_type = t
     s.node_bc_val = np.zeros((s.num_nodes, 4))
     return s
+
+
+def impact_ensemble(count=256, steps=500, nmin=2000, nmax=64000, h=0.01, seed=SEED) -> list:
+    """The batched ensemble: `count` independent perfect-plastic block impacts.
+
+    Per instance (seeded): particle count log-uniform in [nmin, nmax] (a cube of
+    round(N^(1/3))^3 lattice points at psep = h/2), grid of 16..48 nodes per axis sized to
+    the block plus a random margin, rho 7800, K ~ U(1e10, 1.6e11), G ~ U(5e9, 8e10),
+    yield stress ~ U(2e8, 1e9), impact speed ~ U(50, 300) m/s along -x toward a grid_vx
+    wall (x <= 2h); each instance keeps its own CFL time step.
+    """
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for i in range(count):
+        n = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        side = max(2, int(round(n ** (1.0 / 3.0))))
+        margin = int(rng.integers(0, 9))
+        n_nodes = int(min(48, max(16, -(-side // 2) + 10 + margin)))
+        K = float(rng.uniform(1e10, 1.6e11))
+        G = float(rng.uniform(5e9, 8e10))
+        sy = float(rng.uniform(2e8, 1e9))
+        v = float(rng.uniform(50.0, 300.0))
+        sc = impact_block(h=h, n_nodes=n_nodes, count=side ** 3, K=K, G=G, sy=sy, v=v, model=3, seed=seed + i)
+        sc.name = f"impact_{i}"
+        sc.steps = steps
+        sc.meta.update(index=i, K=K, G=G, sy=sy, v=v)
+        scenes.append(sc)
+    return scenes

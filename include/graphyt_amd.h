@@ -175,6 +175,12 @@ int  gy_synchronize(gy_ctx *ctx);
 /* Download particle fields (FP32 device state widened to FP64) in the caller's order. */
 int  gy_download(gy_ctx *ctx, gy_particles *host);
 
+/* Download the node fields of the last step (Nodes::getMassArr / getVelArr /
+ * getForceArr, Nodes.cpp:146-149): mass (n_nodes), centre-of-mass velocity (3 n_nodes)
+ * and force (3 n_nodes) as m * a, with a the nodal acceleration after the velocity BCs
+ * (the reference keeps the force from before them). Any pointer may be NULL. */
+int  gy_download_nodes(gy_ctx *ctx, int32_t n_nodes, double *mass, double *vel, double *force);
+
 /* FP64 mass / momentum / energy audit reduction on the device. */
 int  gy_audit(gy_ctx *ctx, gy_audit_out *out);
 
@@ -188,6 +194,27 @@ int  gy_debug_binning(gy_ctx *ctx, int32_t *keys, int32_t *counts, int32_t *offs
  * step uses: counts, exclusive offsets and order (order[i] = index of the i-th key). */
 int  gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys,
                       int32_t *counts, int32_t *offsets, int32_t *order);
+
+/* ---- batched ensembles ----------------------------------------------------------
+ * No counterpart in the reference, which runs one Solver per problem: `ninst`
+ * independent problems advanced together by every gy_iterate / gy_iterate_n call, in
+ * the same launches (one persistent particle kernel over the tiles of all instances,
+ * one captured graph per step). Instance i has grid grids[i], the nmat[i] materials that
+ * follow the previous instances' in `mats`, and parameters params[i] (its own dt and
+ * gravity). Material ids in a particle view are local to its instance. */
+int  gy_create_batch(gy_ctx **out, int32_t ninst, const gy_grid *grids, const gy_material *mats,
+                     const int32_t *nmat, const gy_params *params);
+int  gy_instance_count(const gy_ctx *ctx);
+int  gy_set_node_bc_inst(gy_ctx *ctx, int32_t inst, int32_t n_nodes, const int32_t *type, const double *val);
+int  gy_upload_batch(gy_ctx *ctx, int32_t ninst, const gy_particles *views);
+int  gy_download_inst(gy_ctx *ctx, int32_t inst, gy_particles *host);
+double gy_get_dt_inst(const gy_ctx *ctx, int32_t inst);
+
+/* Autotuner (no counterpart in the reference): times `trial_steps` steps for each
+ * candidate workgroup count of the persistent particle kernel (1, 2, 3, 4 and 6 per
+ * compute unit), keeps the fastest, and restores the particle state it started from.
+ * Reports the choice and its time per step. */
+int  gy_autotune(gy_ctx *ctx, int32_t trial_steps, int64_t *best_persist_wgs, double *best_ms_per_step);
 
 /* Options: see GY_OPT_*. */
 #define GY_OPT_USE_GRAPH   1   /* replay gy_iterate_n through a captured hipGraph (default 1) */
