@@ -148,6 +148,44 @@ def bench_ensemble(args) -> int:
     return 0
 
 
+def bench_surrogate(args) -> int:
+    """The learned return map against the exact FP32 device map on the column's constitutive
+    samples: 20 pairs of consecutive 1,024,000-particle states (20.5M samples) train the
+    network on the GPU; a held-out pair (step 700) is mapped by both. value = exact-map
+    kernel time / surrogate kernel time (above 1: the surrogate is faster)."""
+    from graphyt_amd import surrogate as S, workloads as W
+    sc = W.dp_column()
+    mat = sc.mats[0]
+    t0 = time.perf_counter()
+    train = S.collect(sc, pairs=20, gap=40, start=30)
+    held = S.collect(sc, pairs=1, gap=10, start=700)
+    log(f"samples: {len(train)} training, {len(held)} held out ({time.perf_counter() - t0:.1f} s)")
+    sur = S.Surrogate(mat, seed=0)
+    t1 = time.perf_counter()
+    st = sur.fit(train, epochs=12)
+    t_fit = time.perf_counter() - t1
+    ev = S.evaluate(sur, held, reps=max(3, min(args.steps, 20)))
+    log("fit:", st, f"{t_fit:.1f} s", "eval:", ev)
+    out = {
+        "metric": METRIC, "value": round(ev["speedup"], 4), "unit": "x (exact-map time / surrogate time)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(ev["surrogate_ms"], 4),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "bf16 matrix cores, f32 accumulate",
+        "data": "synthetic (constitutive samples logged from GPU column runs)",
+        "config": {"workload": "dp_surrogate: 13-64-64-64-7 SiLU return-map network, column material phi 30",
+                   "training_samples": int(len(train)), "yielding_training_samples": st["yielding"],
+                   "held_out_samples": int(len(held)), "epochs": 12, "seed": W.SEED,
+                   "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+        "exact_ms": round(ev["exact_ms"], 4), "surrogate_ms": round(ev["surrogate_ms"], 4),
+        "stress_rel_l2_yielding": ev["stress_rel_l2_yielding"], "epbar_rel_l2_yielding": ev["epbar_rel_l2_yielding"],
+        "network_samples": ev["network"], "fallback_samples": ev["fallback"],
+        "train_seconds": round(t_fit, 2), "final_loss": st["final_loss"],
+    }
+    print(json.dumps(out))
+    sur.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,8 +195,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
-    ap.add_argument("--workload", choices=["dp_column", "ensemble"], default="dp_column",
-                    help="dp_column: the 1M-particle column (default); ensemble: 256 batched impacts")
+    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate"], default="dp_column",
+                    help="dp_column: the 1M-particle column (default); ensemble: 256 batched impacts; "
+                         "surrogate: learned vs exact Drucker-Prager return map")
     ap.add_argument("--autotune", action="store_true", help="run gy_autotune before timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
@@ -178,6 +217,8 @@ def main():
     from graphyt_amd import runtime, workloads as W
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     sc = W.dp_column()
     log(f"workload dp_column: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
     ctx = runtime.Context(sc, graph=True)

@@ -359,6 +359,8 @@ int build_graph(gy_ctx *c) {
 
 }  // namespace
 
+MatDev gy::material_constants(const gy_material &m) { return make_mat(m); }
+
 // ---------------------------------------------------------------------------------------
 extern "C" {
 

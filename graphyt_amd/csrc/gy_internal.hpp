@@ -17,6 +17,8 @@
 #pragma once
 #include <cstdint>
 
+struct gy_material;
+
 namespace gy {
 
 constexpr int TILE = 4;                    // cells per tile edge
@@ -162,6 +164,9 @@ enum KernelId : int {
   K_KEYS = 0, K_SCAN_PARTIAL, K_SCAN_FINAL, K_PLACE, K_P2G, K_GRID, K_G2P2G, K_ADVANCE,
   K_AUDIT, K_ORDER, K_COUNT_
 };
+
+// Device material constants of a gy_material (defined in gy_ctx.hip).
+MatDev material_constants(const gy_material &m);
 
 // Host-side launchers (defined in gy_kernels.hip). Every kernel reads the step number and
 // the buffer parity from Ctrl, so a captured sequence replays correctly step after step.

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gy_internal.hpp"
+#include "gy_models.hpp"
 
 namespace gy {
 
@@ -83,237 +84,6 @@ __device__ __forceinline__ Axis gimp3(float u, float lp, float inv_lp, float inv
   A.g[1] = -G1 * inv_h;
   A.g[2] = -G2 * inv_h;
   return A;
-}
-
-// ---------------------------------------------------------------------------------------
-// Constitutive models (FP32), following ConstitutiveModels.cpp operation by operation.
-
-struct PState {
-  float v[3], s[6], e[6], p[6], epb, rho, mass, dmg, smax;
-};
-
-// elastic, ConstitutiveModels.cpp:3-30
-__device__ __forceinline__ void cm_elastic(const MatDev &M, const float er[6], float s[6], float dt) {
-  const float vol = 0.3333f * (er[0] + er[1] + er[2]);
-  const float G2 = 2.0f * M.G, K3 = 3.0f * M.K;
-  s[0] += (G2 * (er[0] - vol) + K3 * vol) * dt;
-  s[1] += (G2 * (er[1] - vol) + K3 * vol) * dt;
-  s[2] += (G2 * (er[2] - vol) + K3 * vol) * dt;
-  s[3] += (G2 * er[3]) * dt;
-  s[4] += (G2 * er[4]) * dt;
-  s[5] += (G2 * er[5]) * dt;
-}
-
-// newtonianLiquid, ConstitutiveModels.cpp:33-64
-__device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6], const float e[6],
-                                             float s[6], float dt) {
-  const float vol = 0.33333f * (er[0] + er[1] + er[2]);
-  const float P = M.K * (e[0] + e[1] + e[2]);
-  const float c = M.visc * (2.0f - (2.0f / 3.0f));
-  s[0] = P + c * (er[0] - vol) * dt;
-  s[1] = P + c * (er[1] - vol) * dt;
-  s[2] = P + c * (er[2] - vol) * dt;
-  s[3] = 2.0f * M.visc * er[3] * dt;
-  s[4] = 2.0f * M.visc * er[4] * dt;
-  s[5] = 2.0f * M.visc * er[5] * dt;
-}
-
-// perfectPlasticity, ConstitutiveModels.cpp:131-234
-__device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], float s[6], float p[6],
-                                           float &epb, float dt) {
-  const float G = M.G, K = M.K;
-  const float vol = 0.33f * (er[0] + er[1] + er[2]);
-  float tr[6];
-  tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
-  tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
-  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
-  tr[3] = s[3] + (2.0f * G * er[3]) * dt;
-  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
-  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
-  const float a01 = tr[0] - tr[1], a12 = tr[1] - tr[2], a20 = tr[2] - tr[0];
-  const float q = sqrtf(0.5f * a01 * a01 + 0.5f * a12 * a12 + 0.5f * a20 * a20 + tr[3] * tr[3] +
-                        tr[4] * tr[4] + tr[5] * tr[5]);
-  if (!(q > M.sy)) {
-    for (int k = 0; k < 6; k++) s[k] = tr[k];
-    return;
-  }
-  const float dL = (q - M.sy) / (3.0f * G);
-  const float f = dL * (0.5f / q);
-  float pr[6];
-  pr[0] = f * (2.0f * tr[0] - tr[1] - tr[2]);
-  pr[1] = f * (2.0f * tr[1] - tr[2] - tr[0]);
-  pr[2] = f * (2.0f * tr[2] - tr[0] - tr[1]);
-  pr[3] = f * (6.0f * tr[3]);
-  pr[4] = f * (6.0f * tr[4]);
-  pr[5] = f * (6.0f * tr[5]);
-  const float b01 = pr[0] - pr[1], b12 = pr[1] - pr[2], b20 = pr[2] - pr[0];
-  epb += dt * sqrtf(0.5f * b01 * b01 + 0.5f * b12 * b12 + 0.5f * b20 * b20 + pr[3] * pr[3] +
-                    pr[4] * pr[4] + pr[5] * pr[5]);
-  const float inv_dt = 1.0f / dt;
-  float el[6];
-  for (int k = 0; k < 6; k++) {
-    p[k] += pr[k];
-    el[k] = er[k] - pr[k] * inv_dt;
-  }
-  s[0] += (2.0f * G * (el[0] - vol) + 3.0f * K * vol) * dt;
-  s[1] += (2.0f * G * (el[1] - vol) + 3.0f * K * vol) * dt;
-  s[2] += (2.0f * G * (el[2] - vol) + 3.0f * K * vol) * dt;
-  s[3] += (2.0f * G * el[3]) * dt;
-  s[4] += (2.0f * G * el[4]) * dt;
-  s[5] += (2.0f * G * el[5]) * dt;
-}
-
-// druckerPrager_Damage, ConstitutiveModels.cpp:238-428. The two Newton loops solve
-// piecewise-linear equations; the reference iterates to |F/c| <= 1e-8 or 201 passes, this
-// FP32 form iterates to its fixed point (at most 16 passes) — the same root to rounding.
-__device__ __forceinline__ void cm_drucker_prager(const MatDev &M, const float er[6], float s[6],
-                                                  float p[6], float &epb, float dt) {
-  const float G = M.G, K = M.K;
-  const float vol = 0.3333f * (er[0] + er[1] + er[2]);
-  float tr[6];
-  tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
-  tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
-  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
-  tr[3] = s[3] + (2.0f * G * er[3]) * dt;
-  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
-  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
-  const float I1 = (1.0f / 3.0f) * (tr[0] + tr[1] + tr[2]);
-  const float sxx = tr[0] - I1, syy = tr[1] - I1, szz = tr[2] - I1;
-  const float sxy = tr[3], syz = tr[4], sxz = tr[5];
-  const float J2 = 0.5f * (sxx * sxx + syy * syy + szz * szz + 2.0f * sxy * sxy + 2.0f * syz * syz +
-                           2.0f * sxz * sxz);
-  const float sJ2 = sqrtf(J2);
-  const float F = sJ2 + M.eta * I1 - M.xi * M.c;
-  const bool yield = M.damage ? ((F > 0.0f) && (I1 < 0.0f)) : (F > 0.0f);
-  if (!yield) {
-    for (int k = 0; k < 6; k++) s[k] = tr[k];
-    return;
-  }
-  const float ep0 = epb;
-  float dL = 0.0f;
-  const float slope0 = -G - K * M.eta * M.eta_bar;
-  for (int it = 0; it < 16; it++) {
-    float cv = M.c - (ep0 + M.xi * dL) * M.c_over_ecr;
-    float dcv = -M.c_over_ecr;
-    if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
-    const float Fp = sJ2 - G * dL + M.eta * (I1 - M.eta_bar * K * dL) - M.xi * cv;
-    const float dFp = slope0 - M.xi * M.xi * dcv;
-    const float dLn = dL - Fp / dFp;
-    const bool done = (dLn == dL) || !(dLn == dLn);
-    dL = dLn;
-    if (done) break;
-  }
-  if ((sJ2 - G * dL < 0.0f) && !M.damage) {
-    // apex return (ConstitutiveModels.cpp:347-395)
-    float de = 0.0f;
-    for (int it = 0; it < 16; it++) {
-      float cv = M.c - (ep0 + M.xi_over_eta * de) * M.c_over_ecr;
-      float dcv = -M.c_over_ecr;
-      if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
-      float Gc = cv * M.xi_over_etab, dGc = M.xi_over_eta * dcv * M.xi_over_etab;
-      if (M.apex_guard) {
-        if (cv == 0.0f) Gc = 0.0f;
-        if (dcv == 0.0f) dGc = 0.0f;
-      }
-      const float Gp = Gc - I1 + K * de;
-      const float den = de - Gp / (dGc + K);
-      const bool done = (den == de) || !(den == den);
-      de = den;
-      if (done) break;
-    }
-    if (de < 0.0f) return;  // deps < 0: stress left as it was (ConstitutiveModels.cpp:376-378)
-    const float sm = I1 - K * de;
-    s[0] = sm; s[1] = sm; s[2] = sm; s[3] = 0.0f; s[4] = 0.0f; s[5] = 0.0f;
-    epb += de * M.xi_over_eta;
-    const float pv = de * (M.eta_bar / 3.0f);
-    p[0] += pv; p[1] += pv; p[2] += pv;
-    return;
-  }
-  const float r = 1.0f - G * dL / sJ2;
-  const float sm = I1 - K * M.eta_bar * dL;
-  s[0] = r * sxx + sm;
-  s[1] = r * syy + sm;
-  s[2] = r * szz + sm;
-  s[3] = r * sxy;
-  s[4] = r * syz;
-  s[5] = r * sxz;
-  epb += dL * M.xi;
-  const float hj = 0.5f / J2, pv = M.eta_bar / 3.0f;
-  p[0] += dL * (hj * sxx + pv);
-  p[1] += dL * (hj * syy + pv);
-  p[2] += dL * (hj * szz + pv);
-  p[3] += dL * hj * sxy;
-  p[4] += dL * hj * syz;
-  p[5] += dL * hj * sxz;
-}
-
-// Symmetric 3x3 eigen-decomposition in FP64 by cyclic Jacobi rotations (Golub & Van Loan,
-// Alg. 8.5.3); eigenvectors in the columns of V.
-__device__ void jacobi3(double a[3][3], double V[3][3]) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 12; sweep++) {
-    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
-    const double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
-    if (off <= 1e-36 * dg || off == 0.0) break;
-#pragma unroll
-    for (int pq = 0; pq < 3; pq++) {
-      const int p = (pq == 2) ? 1 : 0, q = (pq == 0) ? 1 : 2;
-      if (a[p][q] == 0.0) continue;
-      const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
-      const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-      const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
-      for (int k = 0; k < 3; k++) {
-        const double akp = a[k][p], akq = a[k][q];
-        a[k][p] = c * akp - sn * akq;
-        a[k][q] = sn * akp + c * akq;
-      }
-      for (int k = 0; k < 3; k++) {
-        const double apk = a[p][k], aqk = a[q][k];
-        a[p][k] = c * apk - sn * aqk;
-        a[q][k] = sn * apk + c * aqk;
-      }
-      for (int k = 0; k < 3; k++) {
-        const double vkp = V[k][p], vkq = V[k][q];
-        V[k][p] = c * vkp - sn * vkq;
-        V[k][q] = sn * vkp + c * vkq;
-      }
-    }
-  }
-}
-
-// gradyKippDamage 3D branch (DamageModels.cpp:71-150), the whole block in FP64.
-__device__ void gk_damage(const MatDev &M, float s[6], float &D, float &smax, float mass, float rho,
-                          float dt) {
-  double a[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
-  const double check = fabs(a[0][0] + a[1][1] + a[2][2] + a[0][1] + a[1][2] + a[0][2]);
-  if (!(check > 1e-3)) return;
-  double V[3][3];
-  jacobi3(a, V);
-  double ev[3] = {a[0][0], a[1][1], a[2][2]};
-  const double smx = fmax(ev[0], fmax(ev[1], fmax(ev[2], 0.0)));
-  smax = (float)smx;
-  if (smx > 1e-3) {
-    const double eff = smx / M.gk_K43;
-    const double vol = (double)mass / (double)rho;
-    const double emin = pow(vol * M.gk_k, -1.0 / M.gk_m);
-    if (eff > emin) {
-      const double rate = M.gk_m3 * M.gk_alpha13 * pow(eff, M.gk_m / 3.0);
-      double Dn = pow(pow((double)D, 1.0 / 3.0) + (double)dt * rate, 3.0);
-      if (Dn > 1.0) Dn = 1.0;
-      D = (float)Dn;
-      for (int k = 0; k < 3; k++)
-        if (ev[k] > 0.0) ev[k] = (1.0 - Dn) * ev[k];
-    }
-  }
-  double o[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < 3; c++) {
-    const double n = sqrt(V[0][c] * V[0][c] + V[1][c] * V[1][c] + V[2][c] * V[2][c]);
-    const double x = V[0][c] / n, y = V[1][c] / n, z = V[2][c] / n, l = ev[c];
-    o[0] += x * l * x; o[1] += y * l * y; o[2] += z * l * z;
-    o[3] += x * l * y; o[4] += y * l * z; o[5] += x * l * z;
-  }
-  for (int k = 0; k < 6; k++) s[k] = (float)o[k];
 }
 
 // Particle BC handlers, TimeIntegration.cpp:177-296 (phase is ignored by the reference).

@@ -21,6 +21,7 @@ _PP = C.POINTER(C.c_void_p)
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "gy_create": (C.c_int, [_PP, C.POINTER(GyGrid), C.POINTER(GyMaterial), C.c_int32, C.POINTER(GyParams)]),
     "gy_set_node_bc": (C.c_int, [_P, C.c_int32, _I32P, _DP]),
@@ -43,6 +44,18 @@ SIGNATURES = {
     "gy_download_inst": (C.c_int, [_P, C.c_int32, C.POINTER(GyParticles)]),
     "gy_get_dt_inst": (C.c_double, [_P, C.c_int32]),
     "gy_autotune": (C.c_int, [_P, C.c_int32, _I64P, _DP]),
+    "gy_mlp_create": (C.c_int, [_PP, C.c_uint64]),
+    "gy_mlp_param_count": (C.c_int, []),
+    "gy_mlp_set_params": (C.c_int, [_P, _FP]),
+    "gy_mlp_get_params": (C.c_int, [_P, _FP]),
+    "gy_mlp_set_norm": (C.c_int, [_P, _FP, _FP, _FP, _FP, _FP]),
+    "gy_mlp_forward": (C.c_int, [_P, _FP, C.c_int64, _FP]),
+    "gy_mlp_gradient": (C.c_int, [_P, _FP, _FP, C.c_int64, _FP, _DP]),
+    "gy_mlp_load_data": (C.c_int, [_P, _FP, _FP, C.c_int64]),
+    "gy_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, _DP]),
+    "gy_dp_map": (C.c_int, [C.POINTER(GyMaterial), _P, C.c_int32, _FP, _FP, C.c_int64, _I64P, _I64P, _DP]),
+    "gy_mlp_last_error": (C.c_char_p, [_P]),
+    "gy_mlp_destroy": (None, [_P]),
     "gy_set_option": (C.c_int, [_P, C.c_int32, C.c_int64]),
     "gy_kernel_count": (C.c_int, []),
     "gy_kernel_name": (C.c_char_p, [C.c_int32]),
@@ -259,6 +272,92 @@ def autotune(ctx, trial_steps: int = 20) -> dict:
     wgs, ms = C.c_int64(), C.c_double()
     ctx._check(ctx.lib.gy_autotune(ctx.h, trial_steps, C.byref(wgs), C.byref(ms)))
     return {"persist_wgs": wgs.value, "ms_per_step": ms.value}
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class Mlp:
+    """The return-map network on the device (gy_mlp_*): 13 -> 64 -> 64 -> 64 -> 7, SiLU,
+    BF16 matrix-core products with FP32 accumulation."""
+    N_IN, N_OUT = 13, 7
+
+    def __init__(self, seed: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.gy_mlp_create(C.byref(h), seed)
+        self.h = h
+        self._check(rc)
+        self.nparam = self.lib.gy_mlp_param_count()
+
+    def _check(self, rc):
+        if rc != GY_OK:
+            msg = self.lib.gy_mlp_last_error(self.h).decode() if self.h else ""
+            raise GyError(rc, msg)
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparam, np.float32)
+        self._check(self.lib.gy_mlp_get_params(self.h, p.ctypes.data_as(_FP)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparam
+        self._check(self.lib.gy_mlp_set_params(self.h, p.ctypes.data_as(_FP)))
+
+    def set_norm(self, in_mean, in_std, in_lo, in_hi, out_scale):
+        a = [_f32(x) for x in (in_mean, in_std, in_lo, in_hi, out_scale)]
+        self._norm = a
+        self._check(self.lib.gy_mlp_set_norm(self.h, *[x.ctypes.data_as(_FP) for x in a]))
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        y = np.zeros((x.shape[0], self.N_OUT), np.float32)
+        self._check(self.lib.gy_mlp_forward(self.h, x.ctypes.data_as(_FP), x.shape[0], y.ctypes.data_as(_FP)))
+        return y
+
+    def gradient(self, x, t):
+        x, t = _f32(x), _f32(t)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.gy_mlp_gradient(self.h, x.ctypes.data_as(_FP), t.ctypes.data_as(_FP), x.shape[0],
+                                             g.ctypes.data_as(_FP), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs=10, batch=4096, lr=1e-3, lr_decay=0.8) -> float:
+        x, t = _f32(x), _f32(t)
+        self._check(self.lib.gy_mlp_load_data(self.h, x.ctypes.data_as(_FP), t.ctypes.data_as(_FP), x.shape[0]))
+        loss = C.c_double()
+        self._check(self.lib.gy_mlp_train(self.h, epochs, batch, lr, lr_decay, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            self.lib.gy_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dp_map(mat, samples, net: "Mlp | None" = None) -> dict:
+    """The Drucker-Prager return map over raw samples [n, 13] on the GPU: exact FP32 map
+    (net None) or the gated surrogate. Returns out [n, 7], counts and the kernel time."""
+    lib = load()
+    x = _f32(samples)
+    y = np.zeros((x.shape[0], 7), np.float32)
+    nn_, nf, ms = C.c_int64(), C.c_int64(), C.c_double()
+    rc = lib.gy_dp_map(C.byref(mat), net.h if net else None, 1 if net else 0, x.ctypes.data_as(_FP),
+                       y.ctypes.data_as(_FP), x.shape[0], C.byref(nn_), C.byref(nf), C.byref(ms))
+    if rc != GY_OK:
+        raise GyError(rc, lib.gy_mlp_last_error(net.h).decode() if net else "gy_dp_map failed")
+    return {"out": y, "network": nn_.value, "fallback": nf.value, "ms": ms.value}
 
 
 def counting_sort(keys, nkeys: int):

@@ -216,6 +216,41 @@ double gy_get_dt_inst(const gy_ctx *ctx, int32_t inst);
  * Reports the choice and its time per step. */
 int  gy_autotune(gy_ctx *ctx, int32_t trial_steps, int64_t *best_persist_wgs, double *best_ms_per_step);
 
+/* ---- learned Drucker-Prager return map -------------------------------------------
+ * No counterpart in the reference. A 13-64-64-64-7 SiLU network (BF16 matrix-core
+ * products, FP32 accumulation) trained on the GPU on samples of the return map
+ * (druckerPrager_Damage, ConstitutiveModels.cpp:238-428). Sample layout: inputs
+ * [strain increment xx yy zz xy yz xz, prior stress (same order), accumulated plastic
+ * strain], outputs [new stress, new accumulated plastic strain]. The network works on
+ * normalised values: inputs (x - mean) / std, outputs the plastic correction of the
+ * elastic trial stress and the plastic-strain increment, each divided by out_scale. */
+#define GY_MLP_IN      13
+#define GY_MLP_OUT     7
+#define GY_MLP_PARAMS  10448   /* padded layout: W1 16x64, W2 64x64, W3 64x64, W4 64x16, biases */
+typedef struct gy_mlp gy_mlp;
+int  gy_mlp_create(gy_mlp **out, uint64_t seed);
+int  gy_mlp_param_count(void);
+int  gy_mlp_set_params(gy_mlp *m, const float *params);
+int  gy_mlp_get_params(gy_mlp *m, float *params);
+/* normalisation, and the per-input range [lo, hi] (normalised units) of the gate */
+int  gy_mlp_set_norm(gy_mlp *m, const float *in_mean, const float *in_std, const float *in_lo,
+                     const float *in_hi, const float *out_scale);
+/* network on normalised inputs x[n][13] -> y[n][7] */
+int  gy_mlp_forward(gy_mlp *m, const float *x, int64_t n, float *y);
+/* gradient of mean((y - t)^2) over the n samples w.r.t. the GY_MLP_PARAMS parameters */
+int  gy_mlp_gradient(gy_mlp *m, const float *x, const float *t, int64_t n, float *grad, double *loss);
+/* training set (normalised) kept on the device; Adam over mini-batches */
+int  gy_mlp_load_data(gy_mlp *m, const float *x, const float *t, int64_t n);
+int  gy_mlp_train(gy_mlp *m, int32_t epochs, int32_t batch, double lr, double lr_decay, double *last_loss);
+/* The return map over n raw samples: exact FP32 map (surrogate = 0), or the gated
+ * surrogate (elastic trial exactly; yielding samples in the trained range through the
+ * network, the others through the exact map). Reports how many samples took the network
+ * and the fallback, and the kernel time. */
+int  gy_dp_map(const gy_material *mat, gy_mlp *m, int32_t surrogate, const float *in, float *out, int64_t n,
+               int64_t *n_network, int64_t *n_fallback, double *kernel_ms);
+const char *gy_mlp_last_error(const gy_mlp *m);
+void gy_mlp_destroy(gy_mlp *m);
+
 /* Options: see GY_OPT_*. */
 #define GY_OPT_USE_GRAPH   1   /* replay gy_iterate_n through a captured hipGraph (default 1) */
 #define GY_OPT_PROFILE     2   /* record HIP events around every kernel (default 0)          */
