@@ -19,7 +19,8 @@ using namespace gy;
 namespace {
 
 const char *kKernelNames[K_COUNT_] = {"keys",  "scan_partial", "scan_final", "place",   "p2g",
-                                      "grid",  "g2p2g",        "advance",    "audit",   "order"};
+                                      "grid",  "g2p2g",        "advance",    "audit",   "order",
+                                      "mb_claim", "mb_p2g"};
 
 struct Timed {
   int id;
@@ -233,6 +234,7 @@ int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, con
   I.qscale = (float)(H.dt * (double)(1 << F) / h);
   I.damping = (float)p->damping;
   I.inv_2F = (float)(1.0 / (double)(1 << F));
+  I.mu = (float)p->mu;
   I.nmat = nmat;
   return GY_OK;
 }
@@ -317,6 +319,18 @@ int check_status(gy_ctx *c) {
 // One step after the binning of the current buffer is in place.
 void enqueue_step(gy_ctx *c) {
   void *s = c->stream;
+  if (c->d.multibody) {
+    timed(c, K_GRID, [&] { launch_mb_grid(c->d, s); });
+    timed(c, K_G2P2G, [&] { launch_mb_update(c->d, s); });
+    timed(c, K_ADVANCE, [&] { launch_advance(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+    timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
+    timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
+    timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
+    return;
+  }
   timed(c, K_GRID, [&] { launch_grid(c->d, s); });
   timed(c, K_G2P2G, [&] { launch_g2p2g(c->d, s); });
   timed(c, K_ADVANCE, [&] { launch_advance(c->d, s); });
@@ -332,6 +346,12 @@ void enqueue_init(gy_ctx *c) {
   timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
   timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
   timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+  if (c->d.multibody) {
+    timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
+    timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
+    timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
+    return;
+  }
   timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
 }
 
@@ -441,19 +461,21 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
   if (ninst != (int)c->inst.size())
     return fail(c, GY_ERR_INVALID, "upload of %d instance(s) into a context of %d", ninst, (int)c->inst.size());
   long long total = 0;
+  bool multibody = false;
   for (int k = 0; k < ninst; k++) {
     const gy_particles *in = &views[k];
     if (!in->pos || in->n <= 0) return fail(c, GY_ERR_INVALID, "instance %d: upload needs n > 0 and positions", k);
     const InstHost &H = c->ih[k];
     const int nmat = (int)H.mats.size();
-    // multi-body contact (two velocity fields per node) is not on the single-body path
+    // several objects or rigid materials: the two-body contact step
     const int obj0 = in->obj_id ? in->obj_id[0] : 0;
     for (int i = 0; i < in->n; i++) {
       const int m = in->material ? in->material[i] : 0;
       if (m < 0 || m >= nmat) return fail(c, GY_ERR_INVALID, "instance %d particle %d: material %d out of range", k, i, m);
-      if (H.mats[m].is_rigid) return fail(c, GY_ERR_UNSUPPORTED, "rigid materials need two-body contact (not on this path)");
-      if (in->obj_id && in->obj_id[i] != obj0)
-        return fail(c, GY_ERR_UNSUPPORTED, "more than one object ID needs two-body contact (not on this path)");
+      if (H.mats[m].is_rigid) multibody = true;
+      if (in->obj_id && (in->obj_id[i] < 0 || in->obj_id[i] > 0xffff))
+        return fail(c, GY_ERR_INVALID, "instance %d particle %d: object id must be in [0, 65535]", k, i);
+      if (in->obj_id && in->obj_id[i] != obj0) multibody = true;
       if (in->bc_type && (in->bc_type[i] < 0 || in->bc_type[i] > 127))
         return fail(c, GY_ERR_INVALID, "instance %d particle %d: bad BC type", k, i);
     }
@@ -554,6 +576,14 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     pstart += n;
   }
   d.kmask = kmask;
+  d.multibody = multibody ? 1 : 0;
+  if (multibody) {
+    if ((rc = dalloc(c, &d.acc2, (size_t)c->nnodes * MB_ACC2_W))) return rc;
+    if ((rc = dalloc(c, &d.nrm, (size_t)c->nnodes * 6))) return rc;
+    if ((rc = dalloc(c, &d.claim, (size_t)c->nnodes))) return rc;
+    if ((rc = dalloc(c, &d.velb, (size_t)c->nnodes * MB_VELB_W))) return rc;
+    if ((rc = dalloc(c, &d.cellobj, (size_t)d.nkeys))) return rc;
+  }
   HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(d.pbc_val, pbc.data(), pbc.size() * 4, hipMemcpyHostToDevice, c->stream));
   if ((rc = push_inst(c))) return rc;
@@ -930,6 +960,7 @@ void gy_destroy(gy_ctx *c) {
   dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
   dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
   dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.pcore); dfree(d.partial);
+  dfree(d.acc2); dfree(d.nrm); dfree(d.claim); dfree(d.velb); dfree(d.cellobj);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -83,7 +83,7 @@ struct InstDesc {
   float gravity[3];
   float damping;
   float inv_2F;      // 2^-F
-  float pad1;
+  float mu;          // contact friction coefficient (Parameters::mu)
 };
 
 // Per-material constants, derived on the host in FP64 (ConstitutiveModels.cpp,
@@ -157,12 +157,22 @@ struct DevPtrs {
   int nblocks_scan;
   int kmask;             // model-set bits of the fused kernel specialization
   int persist_wgs;       // workgroups of the persistent particle kernel (multiple of 8)
+  // two-body contact (more than one object or rigid materials); see the MB kernels
+  int multibody;
+  float *acc2;           // MB_ACC2_W per canonical node: body 1 and 2 (m, p, f, rigid mass)
+  float *nrm;            // 6 per node: body 1 and body 2 normal sums (sum gradN * V)
+  int4 *claim;           // per node: object of body 1, object of body 2, rigid flags
+  float *velb;           // MB_VELB_W per node: v1, a1, v2, a2 after the contact solve
+  int4 *cellobj;         // per key: first and second object in the cell, rigid flags
 };
+
+constexpr int MB_ACC2_W = 16;   // per canonical node: [m p3 f3 mr] for body 1, then body 2
+constexpr int MB_VELB_W = 12;   // per node: v1 a1 v2 a2
 
 // Kernel identifiers for per-kernel timing.
 enum KernelId : int {
   K_KEYS = 0, K_SCAN_PARTIAL, K_SCAN_FINAL, K_PLACE, K_P2G, K_GRID, K_G2P2G, K_ADVANCE,
-  K_AUDIT, K_ORDER, K_COUNT_
+  K_AUDIT, K_ORDER, K_MB_CLAIM, K_MB_P2G, K_COUNT_
 };
 
 // Device material constants of a gy_material (defined in gy_ctx.hip).
@@ -183,6 +193,12 @@ void launch_g2p2g(const DevPtrs &d, void *stream);
 void launch_advance(const DevPtrs &d, void *stream);
 void launch_audit(const DevPtrs &d, void *stream);
 void launch_stable_order(const DevPtrs &d, int *order, void *stream);
+// two-body contact step (Grid.cpp:9-308, Extrapolation.cpp:166-367): order of one step is
+// mb_grid, mb_update, advance, scan_partial, scan_final, place, mb_claim, p2g_only, mb_p2g
+void launch_mb_grid(const DevPtrs &d, void *stream);
+void launch_mb_update(const DevPtrs &d, void *stream);
+void launch_mb_claim(const DevPtrs &d, void *stream);   // cell objects + node body claims
+void launch_mb_p2g(const DevPtrs &d, void *stream);
 int scan_items_per_block();
 
 }  // namespace gy

@@ -18,6 +18,8 @@
 // Binning replaces sortParticlesByCellID_CPU (Extrapolation.cpp:370-503) and runs every step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gy_internal.hpp"
 #include "gy_models.hpp"
 
@@ -535,10 +537,15 @@ __device__ __forceinline__ int wave_hist_rank(int *count, int key) {
 // Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
 // constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
 // state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
-template <bool UPDATE, int MASK>
+// MB (two-body contact): G2P reads the velocity field of the body slot the particle's
+// object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots' objects;
+// Extrapolation.cpp:288-367), and no record is written (P2G runs after the re-binning).
+template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
                                                int i, int slot, float (*s_vel)[NODES_G2P], const MatDev *s_mat,
-                                               int par, float (*s_rec)[CHUNK], int j) {
+                                               int par, float (*s_rec)[CHUNK], int j,
+                                               float (*s_vel2)[NODES_G2P] = nullptr,
+                                               int (*s_obj)[NODES_G2P] = nullptr) {
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
   const long long cap = d.cap;
@@ -591,12 +598,25 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         for (int ii2 = 0; ii2 < 3; ii2++) {
           const int n = row + ii2;
           const float wx = W[0].w[ii2], gx = W[0].g[ii2];
+          if (MB) {
+            const int ob = tag_obj(tag);
+            const int sl = (s_obj[0][n] == ob) ? 0 : (s_obj[1][n] == ob) ? 1 : 2;
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const float vn = s_vel[c][n];
-            S0[c] += wx * vn;
-            S1[c] += gx * vn;
-            Sa[c] += wx * s_vel[3 + c][n];
+            for (int c = 0; c < 3; c++) {
+              const float vn = sl == 0 ? s_vel[c][n] : sl == 1 ? s_vel2[c][n] : 0.0f;
+              const float an = sl == 0 ? s_vel[3 + c][n] : sl == 1 ? s_vel2[3 + c][n] : 0.0f;
+              S0[c] += wx * vn;
+              S1[c] += gx * vn;
+              Sa[c] += wx * an;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              const float vn = s_vel[c][n];
+              S0[c] += wx * vn;
+              S1[c] += gx * vn;
+              Sa[c] += wx * s_vel[3 + c][n];
+            }
           }
         }
 #pragma unroll
@@ -718,6 +738,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
   }
 
+  if (MB) return knew;
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
@@ -973,6 +994,384 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Two-body contact (contexts holding several objects or rigid materials).
+//
+// Per node the reference keeps centre-of-mass fields and two body slots: the first object
+// a node's gather meets claims slot 1, the first different object slot 2
+// (Extrapolation.cpp:191-208); P2G adds every particle to the centre-of-mass fields and
+// to its object's slot (Extrapolation.cpp:210-280); the nodal solve corrects approaching
+// bodies (Grid.cpp:9-245, 247-308); G2P reads the slot of the particle's object
+// (Extrapolation.cpp:288-367). Here the step is unfused: mb_grid, mb_update (G2P +
+// update + store + new key), advance, binning, mb_claim (cell objects, then node claims
+// from the new binning), the centre-of-mass P2G of the tile kernel (k_g2p2g without the
+// update), and mb_p2g for the body fields. A node with one claimed body has body fields
+// equal to its centre-of-mass fields (every particle its window sees is of that object),
+// so body fields and normals are scattered only to nodes with two claimed bodies; rigid
+// particles also scatter their mass to their body's slot everywhere (gravity acts on the
+// body fields of non-rigid mass only).
+
+// Objects present in each cell of the active tiles: the two smallest object ids and
+// whether each has a rigid particle there. A cell's particles are met in ascending object
+// id when a node claims slots (the reference's order inside a cell is its sort order).
+__global__ __launch_bounds__(WG) void k_mb_cells(DevPtrs d) {
+  const int nact = d.ctrl->active_tiles;
+  const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
+  for (int ta = blockIdx.x * (WG / TILE3) + (threadIdx.x / TILE3); ta < nact; ta += gridDim.x * (WG / TILE3)) {
+    const int t = d.active[ta];
+    const InstDesc &I = d.inst[d.tile_inst[t]];
+    const int k = t * TILE3 + (threadIdx.x % TILE3);
+    int a = 0x7fffffff, b = 0x7fffffff, ra = 0, rb = 0;
+    for (int j = d.offsets[k]; j < d.offsets[k + 1]; j++) {
+      const uint32_t tag = A[F_TAG * d.cap + d.tmp[j]];
+      const int ob = tag_obj(tag), rg = d.mats[I.mat_base + tag_mat(tag)].rigid ? 1 : 0;
+      if (ob == a) ra |= rg;
+      else if (ob < a) { b = a; rb = ra; a = ob; ra = rg; }
+      else if (ob == b) rb |= rg;
+      else if (ob < b) { b = ob; rb = rg; }
+    }
+    d.cellobj[k] = make_int4(a == 0x7fffffff ? -1 : a, b == 0x7fffffff ? -1 : b, ra, rb);
+  }
+}
+
+// Body claims of every node a particle of an active tile can reach (the core region of
+// the tile): the node's window is the 4x4x4 cells nidx-2..nidx+1 with linear cell ids
+// and only a range check (Extrapolation.cpp:36-66, so ids past a row end continue on the
+// next row), met in ascending (k, j, i).
+__global__ __launch_bounds__(WG) void k_mb_claim(DevPtrs d) {
+  const long long nact = d.ctrl->active_tiles;
+  for (long long it = (long long)blockIdx.x * WG + threadIdx.x; it < nact * CORE_NODES;
+       it += (long long)gridDim.x * WG) {
+    const int t = d.active[it / CORE_NODES], ln = (int)(it % CORE_NODES);
+    const InstDesc &I = d.inst[d.tile_inst[t]];
+    const int lt = t - I.tile_base;
+    const int g0 = (lt % I.nt[0]) * TILE + ln % CORE_R, g1 = ((lt / I.nt[0]) % I.nt[1]) * TILE + (ln / CORE_R) % CORE_R,
+              g2 = (lt / (I.nt[0] * I.nt[1])) * TILE + ln / (CORE_R * CORE_R);
+    if (g0 >= I.n[0] || g1 >= I.n[1] || g2 >= I.n[2]) continue;
+    const int Cx = I.n[0] - 1, Cy = I.n[1] - 1, Cz = I.n[2] - 1;
+    const long long ncell = (long long)Cx * Cy * Cz;
+    int b1 = -1, b2 = -1, r1 = 0, r2 = 0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int kz = -2; kz < 2; kz++)
+        for (int jy = -2; jy < 2; jy++)
+          for (int ix = -2; ix < 2; ix++) {
+            const long long lin = (g0 + ix) + (long long)(g1 + jy) * Cx + (long long)(g2 + kz) * Cx * Cy;
+            if (lin < 0 || lin >= ncell) continue;
+            const int c[3] = {(int)(lin % Cx), (int)((lin / Cx) % Cy), (int)(lin / ((long long)Cx * Cy))};
+            if (c[0] > I.n[0] - 3 || c[1] > I.n[1] - 3 || c[2] > I.n[2] - 3) continue;  // no particle there
+            const int k = key_of(I, c);
+            if (d.offsets[k + 1] == d.offsets[k]) continue;
+            const int4 co = d.cellobj[k];
+            if (pass == 0) {
+              const int ob[2] = {co.x, co.y};
+              for (int q = 0; q < 2; q++) {
+                if (ob[q] < 0) continue;
+                if (b1 < 0) b1 = ob[q];
+                else if (b2 < 0 && ob[q] != b1) b2 = ob[q];
+              }
+            } else {
+              r1 |= (co.x == b1 && co.z) || (co.y == b1 && co.w);
+              r2 |= b2 >= 0 && ((co.x == b2 && co.z) || (co.y == b2 && co.w));
+            }
+          }
+    d.claim[I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2)] = make_int4(b1, b2, r1, r2);
+  }
+}
+
+// Body-field P2G of one particle (slot order) at the nodes where two bodies are claimed:
+// body fields to the slot its object holds at the actual node, folded onto the canonical
+// node (rigid particles: unit mass, no gravity in their body's force,
+// Extrapolation.cpp:183, 224-229), normals sum gradN * V per actual node
+// (Extrapolation.cpp:231-234); rigid particles add their mass to their slot's rigid-mass
+// sum at every node. FP32 global atomics.
+__global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
+  const int s = blockIdx.x * WG + threadIdx.x;
+  if (s >= d.np) return;
+  const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
+  const long long cap = d.cap;
+  const int ii = inst_of_slot(d, s);
+  const InstDesc &I = d.inst[ii];
+  const uint32_t tag = A[F_TAG * cap + s];
+  const MatDev &M = d.mats[I.mat_base + tag_mat(tag)];
+  const int obj = tag_obj(tag);
+  int q[3], b[3];
+  bool bad = false;
+  Axis W[3];
+  for (int a = 0; a < 3; a++) {
+    q[a] = (int)A[(F_QX + a) * cap + s];
+    b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
+    W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+  }
+  const float mass = f32(A[F_MASS * cap + s]), rho = f32(A[F_RHO * cap + s]);
+  const float mp = M.rigid ? 1.0f : mass;
+  P2GTerms T;
+  T.m = mp;
+  for (int a = 0; a < 3; a++) T.mv[a] = mp * f32(A[(F_VX + a) * cap + s]);
+  const float V = mp / rho;
+  for (int a = 0; a < 6; a++) T.vs[a] = -V * f32(A[(F_S0 + a) * cap + s]);
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        const float A_ = W[1].w[j] * W[2].w[k];
+        const float w = W[0].w[i] * A_;
+        const float gx = W[0].g[i] * A_, gy = W[0].w[i] * W[1].g[j] * W[2].w[k], gz = W[0].w[i] * W[1].w[j] * W[2].g[k];
+        if (w == 0.0f && gx == 0.0f && gy == 0.0f && gz == 0.0f) continue;
+        const int g[3] = {b[0] + i, b[1] + j, b[2] + k};
+        const long long gi = I.node_base + g[0] + (long long)I.n[0] * (g[1] + (long long)I.n[1] * g[2]);
+        const int4 cl = d.claim[gi];
+        const bool two = cl.y != -1;
+        if (!two && !M.rigid) continue;
+        const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
+        if (sl < 0) continue;
+        int c[3];
+        const long long ci = canon_index(I, g, c);
+        float *ab = d.acc2 + ci * MB_ACC2_W + sl * 8;
+        if (M.rigid) unsafeAtomicAdd(&ab[7], w * T.m);
+        if (!two) continue;
+        float t[7];
+        node_terms(T, w, gx, gy, gz, t);
+        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], t[f]);
+        float *nb = d.nrm + gi * 6 + sl * 3;
+        unsafeAtomicAdd(&nb[0], gx * V);
+        unsafeAtomicAdd(&nb[1], gy * V);
+        unsafeAtomicAdd(&nb[2], gz * V);
+      }
+}
+
+// calculateColoumbFrictionContactNormals (Grid.cpp:247-308) for one body: the contact
+// normal n + min(mu, mu0) * (omega x ...) with the reference's component formulas.
+__device__ __forceinline__ void contact_normal(const float rel[3], const float n[3], float mu, float cn[3]) {
+  float om[3] = {rel[1] * n[2] - rel[2] * n[1], rel[0] * n[2] - rel[2] * n[0], rel[0] * n[1] - rel[1] * n[0]};
+  const float omag = sqrtf(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]) + 0.001f;
+  for (int a = 0; a < 3; a++) om[a] /= omag;
+  const float mu0 = omag / (rel[0] * n[0] + rel[1] * n[1] + rel[2] * n[2]) + 0.001f;
+  const float on[3] = {om[2] * n[1] - om[1] * n[2], om[2] * n[0] - om[0] * n[2], om[1] * n[0] - om[0] * n[1]};
+  const float m = fminf(mu, mu0);
+  for (int a = 0; a < 3; a++) cn[a] = n[a] + m * on[a];
+}
+
+// Two-body nodal solve on the active canonical bricks: centre-of-mass and body sums of the
+// periodic class (P2G folded them onto the canonical node), then per actual node its own
+// claims and normals (Grid.cpp:9-245), BCs on the centre of mass and both bodies
+// (Grid.cpp:967-1031). VEL gets the centre-of-mass velocity p/m, its acceleration and m;
+// VELB the body velocities and accelerations G2P reads.
+__global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
+  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gb = blockIdx.x * (WG / 64) + wave;
+  if (gb >= d.nbricks) return;
+  if (!d.brick_flag[gb]) return;
+  const InstDesc &I = d.inst[d.brick_inst[gb]];
+  const int lb = gb - I.brick_base;
+  const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
+  const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
+  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
+  if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
+    const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
+    // centre-of-mass sums: tile partials of the periodic class and the direct scatter
+    float a7[7] = {0, 0, 0, 0, 0, 0, 0}, ab[MB_ACC2_W];
+    for (int mz = 0; mz < 2; mz++) {
+      if (mz && c2 >= 3) break;
+      for (int my = 0; my < 2; my++) {
+        if (my && c1 >= 3) break;
+        for (int mx = 0; mx < 2; mx++) {
+          if (mx && c0 >= 3) break;
+          gather_partials(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], c2 + mz * I.cper[2], a7);
+        }
+      }
+    }
+    for (int f = 0; f < 7; f++) { a7[f] += d.acc[ci * NODE_W + f]; d.acc[ci * NODE_W + f] = 0.f; }
+    for (int f = 0; f < MB_ACC2_W; f++) { ab[f] = d.acc2[ci * MB_ACC2_W + f]; d.acc2[ci * MB_ACC2_W + f] = 0.f; }
+    const float m = a7[0];
+    const float dt = I.dt, kick = (step == 1) ? 0.5f * dt : dt;
+    float vc[3] = {0.f, 0.f, 0.f}, ac[3] = {0.f, 0.f, 0.f};
+    if (m != 0.0f)
+      for (int k = 0; k < 3; k++) {
+        vc[k] = a7[1 + k] / m;
+        ac[k] = (a7[4 + k] + m * I.gravity[k]) / m;
+      }
+    // body sums: [m, p, f_int, rigid mass]; gravity only on the non-rigid mass
+    float mb[2], pb[2][3], fb[2][3];
+    auto body_sums = [&]() {
+      for (int s2 = 0; s2 < 2; s2++) {
+        mb[s2] = ab[8 * s2];
+        for (int k = 0; k < 3; k++) {
+          pb[s2][k] = ab[8 * s2 + 1 + k];
+          fb[s2][k] = ab[8 * s2 + 4 + k] + (ab[8 * s2] - ab[8 * s2 + 7]) * I.gravity[k];
+        }
+      }
+    };
+    for (int mz = 0; mz < 2; mz++) {
+      if (mz && c2 >= 3) break;
+      const int n2 = c2 + mz * I.cper[2];
+      for (int my = 0; my < 2; my++) {
+        if (my && c1 >= 3) break;
+        const int n1 = c1 + my * I.cper[1];
+        for (int mx = 0; mx < 2; mx++) {
+          if (mx && c0 >= 3) break;
+          const int n0 = c0 + mx * I.cper[0];
+          const long long ni = I.node_base + n0 + (long long)I.n[0] * (n1 + (long long)I.n[1] * n2);
+          const int4 cl = d.claim[ni];
+          body_sums();
+          float nr[2][3];
+          for (int k = 0; k < 3; k++) {
+            nr[0][k] = d.nrm[ni * 6 + k];
+            nr[1][k] = d.nrm[ni * 6 + 3 + k];
+            d.nrm[ni * 6 + k] = 0.f;
+            d.nrm[ni * 6 + 3 + k] = 0.f;
+          }
+          float vb[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, acb[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+          if (cl.y == -1) {  // one body: its fields are the centre-of-mass sums
+            mb[0] = m;
+            for (int k = 0; k < 3; k++) {
+              pb[0][k] = a7[1 + k];
+              fb[0][k] = a7[4 + k] + (m - ab[7]) * I.gravity[k];
+            }
+          }
+          if (m != 0.0f) {
+            for (int s2 = 0; s2 < 2; s2++) {  // normalise (Grid.cpp:19-43)
+              const float mag = sqrtf(nr[s2][0] * nr[s2][0] + nr[s2][1] * nr[s2][1] + nr[s2][2] * nr[s2][2]);
+              for (int k = 0; k < 3; k++) nr[s2][k] = (mag > 1e-8f) ? nr[s2][k] / mag : 0.f;
+            }
+            for (int k = 0; k < 3; k++) nr[1][k] = -nr[0][k];  // Grid.cpp:45-47
+            auto free_body = [&](int s2) {
+              if (mb[s2] == 0.0f) return;
+              for (int k = 0; k < 3; k++) {
+                acb[s2][k] = fb[s2][k] / mb[s2];
+                vb[s2][k] = pb[s2][k] / mb[s2] + acb[s2][k] * kick;
+              }
+            };
+            if (cl.y != -1) {
+              float rel[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, ap[2] = {0.f, 0.f};
+              for (int s2 = 0; s2 < 2; s2++)
+                if (mb[s2] != 0.0f) {
+                  for (int k = 0; k < 3; k++) rel[s2][k] = pb[s2][k] / mb[s2] - vc[k];
+                  ap[s2] = rel[s2][0] * nr[s2][0] + rel[s2][1] * nr[s2][1] + rel[s2][2] * nr[s2][2];
+                }
+              if (ap[0] > 0.01f || ap[1] > 0.01f) {
+                if (cl.z || cl.w) {  // a rigid body: the other takes its velocity (Grid.cpp:78-105)
+                  const int r = cl.z ? 0 : 1, o2 = 1 - r;
+                  if (mb[r] != 0.0f) {
+                    for (int k = 0; k < 3; k++) vb[r][k] = pb[r][k] / mb[r];
+                    if (mb[o2] != 0.0f)
+                      for (int k = 0; k < 3; k++)
+                        vb[o2][k] = pb[o2][k] / mb[o2] - (pb[o2][k] - mb[o2] * vb[r][k]) / mb[o2];
+                  }
+                } else {  // two deformable bodies (Grid.cpp:106-170)
+                  for (int s2 = 0; s2 < 2; s2++) {
+                    if (mb[s2] == 0.0f) continue;
+                    float cn[3];
+                    contact_normal(rel[s2], nr[s2], I.mu, cn);
+                    for (int k = 0; k < 3; k++) {
+                      vb[s2][k] = pb[s2][k] / mb[s2] - ap[s2] * cn[k];
+                      acb[s2][k] = fb[s2][k] / mb[s2] - (ap[s2] * cn[k]) / dt;
+                      vb[s2][k] += acb[s2][k] * kick;
+                    }
+                  }
+                }
+              } else {
+                free_body(0);
+                free_body(1);
+              }
+            } else {
+              free_body(0);
+            }
+          }
+          float vv[3] = {vc[0], vc[1], vc[2]}, aa[3] = {ac[0], ac[1], ac[2]};
+          const int t = d.bc_type[ni];
+          if (t == 1 || t == 2 || t == 11) {  // Grid.cpp:967-982, 1012-1031
+            const float b0 = d.bc_val[2 * ni], b1v = d.bc_val[2 * ni + 1];
+            const int ax0 = (t == 2) ? 1 : 0;
+            const float val0 = b0;
+            vv[ax0] = val0; aa[ax0] = 0.f;
+            for (int s2 = 0; s2 < 2; s2++) { vb[s2][ax0] = val0; acb[s2][ax0] = 0.f; }
+            if (t == 11) {
+              vv[1] = b1v; aa[1] = 0.f;
+              for (int s2 = 0; s2 < 2; s2++) { vb[s2][1] = b1v; acb[s2][1] = 0.f; }
+            }
+          }
+          float4 *vp = reinterpret_cast<float4 *>(d.vel + ni * NODE_W);
+          vp[0] = make_float4(vv[0], vv[1], vv[2], aa[0]);
+          vp[1] = make_float4(aa[1], aa[2], m, 0.f);
+          float *vq = d.velb + ni * MB_VELB_W;
+          for (int k = 0; k < 3; k++) {
+            vq[k] = vb[0][k];
+            vq[3 + k] = acb[0][k];
+            vq[6 + k] = vb[1][k];
+            vq[9 + k] = acb[1][k];
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// G2P + update + constitutive + store + new key on the body-slot velocity fields,
+// persistent over the active tiles (as k_g2p2g, without the P2G phase).
+template <int MASK>
+__global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
+  __shared__ float s_vel[6][NODES_G2P], s_vel2[6][NODES_G2P];
+  __shared__ int s_obj[2][NODES_G2P];
+  __shared__ MatDev s_mat[MAXMAT];
+  __shared__ int s_off[TILE3 + 1];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x;
+  const int grp = blockIdx.x & 7;
+  const int nact = d.ctrl->active_tiles;
+  const int lo = (int)((long long)grp * nact / 8), hi = (int)((long long)(grp + 1) * nact / 8);
+  const int par = d.ctrl->parity;
+  for (;;) {
+    if (tid == 0) {
+      const int jt = lo + atomicAdd(&d.ctrl->next[grp], 1);
+      s_tile = (jt < hi) ? d.active[jt] : -1;
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
+    if (t < 0) break;
+    const int k0 = t * TILE3;
+    const int ii = d.tile_inst[t];
+    const InstDesc I = d.inst[ii];
+    const int lt = t - I.tile_base;
+    const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
+    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
+    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+    for (int n = tid; n < NODES_G2P; n += WG) {
+      const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
+      const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
+      float vq[MB_VELB_W];
+      for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
+      int4 cl = make_int4(-1, -1, 0, 0);
+      if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
+        const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
+        if (d.vel[ni * NODE_W + 6] != 0.0f) {  // G2P skips nodes without mass (Extrapolation.cpp:290)
+          for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
+          cl = d.claim[ni];
+        }
+      }
+      for (int f = 0; f < 6; f++) {
+        s_vel[f][n] = vq[f];
+        s_vel2[f][n] = vq[6 + f];
+      }
+      s_obj[0][n] = cl.x;
+      s_obj[1][n] = cl.y;
+    }
+    __syncthreads();
+    const double tl = d.t_last[ii];
+    const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
+    for (int c0 = p0; c0 < p1; c0 += WG) {
+      const int i = c0 + tid;
+      int knew = -1;
+      if (i < p1)
+        knew = particle_update<true, MASK, true>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, nullptr, 0, s_vel2,
+                                                 s_obj);
+      const int r = wave_hist_rank(d.count, knew);
+      if (knew >= 0) d.rank[i] = r;
+    }
+    __syncthreads();
+  }
+}
+
 // FP64 audit: per block sums of mass, momentum, kinetic energy, a stress-energy proxy and
 // non-finite count.
 constexpr int AUD = 8;
@@ -1078,6 +1477,20 @@ void launch_audit(const DevPtrs &d, void *st) {
   const int nb = blocks(d.np, WG);
   hipLaunchKernelGGL(k_audit, dim3(nb), dim3(WG), 0, S(st), d);
   hipLaunchKernelGGL(k_audit_final, dim3(1), dim3(WG), 0, S(st), d, nb);
+}
+void launch_mb_grid(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+}
+void launch_mb_update(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL((k_mb_update<KM_ALL>), dim3(d.persist_wgs), dim3(WG), 0, S(st), d);
+}
+void launch_mb_claim(const DevPtrs &d, void *st) {
+  const int g = std::max(8, std::min(4096, blocks((long long)d.ntiles * CORE_NODES, WG)));
+  hipLaunchKernelGGL(k_mb_cells, dim3(std::max(8, std::min(4096, blocks(d.ntiles, WG / TILE3)))), dim3(WG), 0, S(st), d);
+  hipLaunchKernelGGL(k_mb_claim, dim3(g), dim3(WG), 0, S(st), d);
+}
+void launch_mb_p2g(const DevPtrs &d, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_mb_p2g, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
 void launch_stable_order(const DevPtrs &d, int *order, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_stable_order, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d, order);

@@ -233,3 +233,42 @@ def impact_ensemble(count=256, steps=500, nmin=2000, nmax=64000, h=0.01, seed=SE
         sc.meta.update(index=i, K=K, G=G, sy=sy, v=v)
         scenes.append(sc)
     return scenes
+
+
+def gk_disk(h=0.001, R=0.05, thickness=0.02, v=0.1, steps=2000) -> Scene:
+    """Brazilian-disk compression with Grady-Kipp damage (after Examples/brazillian3D.py).
+
+    Domain 0.12 x 0.14 x 0.04 m; a disk of radius R and the given thickness (axis along
+    z) at psep = h/2, object 1, elastic (model 1) with damage: rho 2600, K 3e10, G 2e10,
+    crack speed 0.4 sqrt(G/rho), Weibull m 9, k 1e30. Two rigid platens (objects 2 and 3,
+    material 1 = rigid with the reference's default density 1000) 10 mm wide and 3 mm
+    thick touch the top and bottom of the disk and move at -v and +v along y. No gravity,
+    no nodal BCs. dt by the CFL rule over both materials (min rho 1000, Solver.cpp:15-32).
+    """
+    psep = h / 2
+    L = tuple(exact_length(int(round(x / h)) + 1, h) for x in (0.12, 0.14, 0.04))
+    cx, cy, cz = 0.06, 0.07, 0.02
+    nx, ny, nz = (int(round(2 * R / psep)), int(round(2 * R / psep)), int(round(thickness / psep)))
+    box = lattice((cx - R, cy - R, cz - thickness / 2), (nx, ny, nz), psep)
+    disk = box[(box[:, 0] - cx) ** 2 + (box[:, 1] - cy) ** 2 <= R * R]
+    w, tp = 0.01, 0.003
+    cnt = (int(round(w / psep)), int(round(tp / psep)), nz)
+    top = lattice((cx - w / 2, cy + R, cz - thickness / 2), cnt, psep)
+    bot = lattice((cx - w / 2, cy - R - tp, cz - thickness / 2), cnt, psep)
+    pos = np.concatenate([disk, top, bot])
+    n1, n2 = len(disk), len(top)
+    obj = np.concatenate([np.full(n1, 1), np.full(n2, 2), np.full(len(bot), 3)]).astype(np.int32)
+    mat = np.concatenate([np.zeros(n1), np.ones(2 * n2)]).astype(np.int32)
+    vel = np.zeros_like(pos)
+    vel[obj == 2, 1] = -v
+    vel[obj == 3, 1] = v
+    G, rho = 2e10, 2600.0
+    rock = material(model=1, bulk_mod=3e10, shear_mod=G, density=rho, is_damage=True, damage_model=1,
+                    crack_speed=0.4 * math.sqrt(G / rho), damage_m=9.0, damage_k=1e30)
+    platen = material(is_rigid=True)
+    s = Scene("gk_disk", L, h, psep, [rock, platen], steps=steps)
+    s.particles = _particles(pos, vel, material=mat, obj=obj)
+    s.node_bc_type = np.zeros(s.num_nodes, np.int32)
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    s.meta.update(disk=n1, platen=n2)
+    return s
