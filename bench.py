@@ -148,6 +148,45 @@ def bench_ensemble(args) -> int:
     return 0
 
 
+def bench_gk_disk(args) -> int:
+    """Brazilian-disk compression with Grady-Kipp damage (1,266,720 particles: disk plus two
+    rigid platens, two-body contact step, FP64 eig3 per stressed particle)."""
+    from graphyt_amd import runtime, workloads as W
+    sc = W.gk_disk()
+    log(f"workload gk_disk: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
+    ctx = runtime.Context(sc, graph=True)
+    ctx.run(args.warmup)
+    ctx.synchronize()
+    ctx.set_option(runtime.OPT_PROFILE, 1)
+    ctx.reset_kernel_stats()
+    t0 = time.perf_counter()
+    ctx.run(args.steps, step0=args.warmup)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.set_option(runtime.OPT_PROFILE, 0)
+    t0 = time.perf_counter()
+    ctx.run(args.steps, step0=args.warmup + args.steps)
+    ctx.synchronize()
+    wall_graph = time.perf_counter() - t0
+    audit = ctx.audit()
+    best = sc.n * args.steps / min(wall, wall_graph)
+    out = {
+        "metric": METRIC, "value": round(best, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * sc.n / best, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (eig3 and damage f64)",
+        "data": "synthetic",
+        "config": {"workload": "gk_disk: Brazilian disk R 0.05 t 0.02, h 0.001, rigid platens +-0.1 m/s",
+                   "particles": sc.n, "nodes": list(sc.dims), "dt": sc.resolved_dt(), "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in stats.items()},
+        "audit_nonfinite": audit["nonfinite"],
+    }
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
 def bench_surrogate(args) -> int:
     """The learned return map against the exact FP32 device map on the column's constitutive
     samples: 20 pairs of consecutive 1,024,000-particle states (20.5M samples) train the
@@ -195,7 +234,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
-    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate"], default="dp_column",
+    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate", "gk_disk"], default="dp_column",
                     help="dp_column: the 1M-particle column (default); ensemble: 256 batched impacts; "
                          "surrogate: learned vs exact Drucker-Prager return map")
     ap.add_argument("--autotune", action="store_true", help="run gy_autotune before timing")
@@ -219,6 +258,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "surrogate":
         return bench_surrogate(args)
+    if args.workload == "gk_disk":
+        return bench_gk_disk(args)
     sc = W.dp_column()
     log(f"workload dp_column: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
     ctx = runtime.Context(sc, graph=True)

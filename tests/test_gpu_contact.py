@@ -85,3 +85,21 @@ def test_contact_graph_replay_matches_eager():
         g.close()
     for k, tol in (("pos", 1e-7), ("vel", 1e-5), ("stress", 1e-5)):
         assert rel(outs[1][k], outs[0][k]) <= tol, k
+
+
+def test_brazilian_disk_with_rigid_platens(oracle_mod):
+    """The Grady-Kipp disk between two rigid platens (three objects), scaled to h = 0.004
+    (19,960 particles), 300 steps: contact, rigid override and the FP64 eig3 path.
+    Velocity and stress follow the oracle to 3e-4: at this resolution the platens touch the
+    disk on a single node row, where the slot claims decide the contact response."""
+    sc = W.gk_disk(h=0.004, steps=300)
+    g = runtime.Context(sc)
+    g.run(300)
+    a = g.download(["pos", "vel", "stress", "smax"])
+    g.close()
+    o = oracle_mod.CpuSim(sc, "restatement")
+    o.run(300)
+    b = o.get(["pos", "vel", "stress", "smax"])
+    assert (b["smax"] > 0).any()
+    for k, tol in (("pos", 1e-6), ("vel", 3e-4), ("stress", 3e-4), ("smax", 1e-3)):
+        assert rel(a[k], b[k]) <= tol, k
