@@ -322,8 +322,7 @@ void enqueue_step(gy_ctx *c) {
   if (c->d.multibody) {
     timed(c, K_GRID, [&] { launch_mb_grid(c->d, s); });
     timed(c, K_G2P2G, [&] { launch_mb_update(c->d, s); });
-    timed(c, K_ADVANCE, [&] { launch_advance(c->d, s); });
-    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
     timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
     timed(c, K_PLACE, [&] { launch_place(c->d, s); });
     timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
@@ -333,8 +332,7 @@ void enqueue_step(gy_ctx *c) {
   }
   timed(c, K_GRID, [&] { launch_grid(c->d, s); });
   timed(c, K_G2P2G, [&] { launch_g2p2g(c->d, s); });
-  timed(c, K_ADVANCE, [&] { launch_advance(c->d, s); });
-  timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+  timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });  // with the step bookkeeping
   timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
   timed(c, K_PLACE, [&] { launch_place(c->d, s); });
 }

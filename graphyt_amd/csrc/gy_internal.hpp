@@ -180,21 +180,21 @@ MatDev material_constants(const gy_material &m);
 
 // Host-side launchers (defined in gy_kernels.hip). Every kernel reads the step number and
 // the buffer parity from Ctrl, so a captured sequence replays correctly step after step.
-// Order of one step: grid, g2p2g, advance, scan_partial, scan_final, place.
+// Order of one step: grid, g2p2g, scan_partial (with the advance bookkeeping), scan_final,
+// place.
 void launch_set_ctrl(const DevPtrs &d, int step, double t, int reset, void *stream);
 void launch_keys(const DevPtrs &d, void *stream);
 void launch_hist(const DevPtrs &d, void *stream);  // histogram of key[0] (standalone sort)
-void launch_scan_partial(const DevPtrs &d, void *stream);
+void launch_scan_partial(const DevPtrs &d, void *stream, int advance = 0);  // advance: + step bookkeeping
 void launch_scan_final(const DevPtrs &d, void *stream);
 void launch_place(const DevPtrs &d, void *stream);
 void launch_p2g_only(const DevPtrs &d, void *stream);
 void launch_grid(const DevPtrs &d, void *stream);
 void launch_g2p2g(const DevPtrs &d, void *stream);
-void launch_advance(const DevPtrs &d, void *stream);
 void launch_audit(const DevPtrs &d, void *stream);
 void launch_stable_order(const DevPtrs &d, int *order, void *stream);
 // two-body contact step (Grid.cpp:9-308, Extrapolation.cpp:166-367): order of one step is
-// mb_grid, mb_update, advance, scan_partial, scan_final, place, mb_claim, p2g_only, mb_p2g
+// mb_grid, mb_update, scan_partial (+ advance), scan_final, place, mb_claim, p2g_only, mb_p2g
 void launch_mb_grid(const DevPtrs &d, void *stream);
 void launch_mb_update(const DevPtrs &d, void *stream);
 void launch_mb_claim(const DevPtrs &d, void *stream);   // cell objects + node body claims

@@ -1,6 +1,6 @@
 // gy_kernels.hip — HIP kernels (gfx950) of the MPM time step.
 //
-// One step is six launches on one stream:
+// One step is five launches on one stream:
 //   k_grid      nodal solve on the active canonical bricks: gather the P2G tile partials
 //               of every node of the periodic class, v = p/m, a = f/m + g, body-1 kick
 //               (half at step 1), BCs, fan-out to every mirror node
@@ -11,10 +11,11 @@
 //               region with a one-node halo, written out as per-tile FP32 partial sums
 //               with plain stores (Extrapolation.cpp:31-367, TimeIntegration.cpp:11-296,
 //               ConstitutiveModels.cpp, DamageModels.cpp)
-//   k_advance   step/parity/time bookkeeping on the device
-//   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets)
-//   k_place     key-grouped slot list; the stable within-cell order (ascending slot) is
-//               resolved where the list is consumed
+//   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets); the
+//               first also does the step/parity/time bookkeeping
+//   k_place     cell-grouped slot list (within a cell in histogram order; the stable
+//               order, ascending slot, is computed only for gy_debug_binning and
+//               gy_counting_sort)
 // Binning replaces sortParticlesByCellID_CPU (Extrapolation.cpp:370-503) and runs every step.
 #include <hip/hip_runtime.h>
 
@@ -121,19 +122,6 @@ __global__ void k_set_ctrl(DevPtrs d, int step, double t, int reset) {
   }
 }
 
-__global__ void k_advance(DevPtrs d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.ninst) {
-    d.t_last[i] = d.t_cur[i];
-    d.t_cur[i] += (double)d.inst[i].dt;
-  }
-  if (i == 0) {
-    d.ctrl->step += 1;
-    d.ctrl->parity ^= 1;
-    d.ctrl->epoch += 1;
-  }
-}
-
 __device__ __forceinline__ int inst_of_slot(const DevPtrs &d, int s) {
   int lo = 0, hi = d.ninst - 1;
   while (lo < hi) {
@@ -212,7 +200,20 @@ __device__ __forceinline__ void load16(const int *p, int c[16]) {
   }
 }
 
-__global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d) {
+// With `advance`, block 0 also performs the step bookkeeping: times, step, parity and
+// epoch (no block of this kernel reads them; the kernels after it see the new values).
+__global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
+  if (advance && blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < d.ninst; i += WG) {
+      d.t_last[i] = d.t_cur[i];
+      d.t_cur[i] += (double)d.inst[i].dt;
+    }
+    if (threadIdx.x == 0) {
+      d.ctrl->step += 1;
+      d.ctrl->parity ^= 1;
+      d.ctrl->epoch += 1;
+    }
+  }
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   if (base < d.nkeys) {
@@ -510,6 +511,12 @@ constexpr bool P2G_PAIRS = false;
 // P2G record of one particle, handed from the update phase to the scatter phase in LDS:
 // fixed-point position (3), m, m*v (3), -V*sigma (6).
 constexpr int NREC = 13;
+// Records in LDS: three 16-byte words (q0 q1 q2 m | mv0 mv1 mv2 vs0 | vs1 vs2 vs3 vs4) and
+// one more float (vs5) per record.
+struct RecLds {
+  float4 w[CHUNK][3];
+  float t[CHUNK];
+};
 
 // Rank of a lane's key among the equal keys of its wave, with one global atomic per run
 // of equal keys (lanes hold consecutive entries of a cell-grouped list, so equal keys
@@ -537,14 +544,16 @@ __device__ __forceinline__ int wave_hist_rank(int *count, int key) {
 // Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
 // constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
 // state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
+// Node velocity and acceleration of the staged G2P region: s_vel[n] = {(vx, vy, vz, ax),
+// (ay, az, -, -)}, two 16-byte LDS reads per node.
 // MB (two-body contact): G2P reads the velocity field of the body slot the particle's
 // object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots' objects;
 // Extrapolation.cpp:288-367), and no record is written (P2G runs after the re-binning).
 template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
-                                               int i, int slot, float (*s_vel)[NODES_G2P], const MatDev *s_mat,
-                                               int par, float (*s_rec)[CHUNK], int j,
-                                               float (*s_vel2)[NODES_G2P] = nullptr,
+                                               int i, int slot, const float4 (*s_vel)[2], const MatDev *s_mat,
+                                               int par, RecLds *s_rec, int j,
+                                               const float4 (*s_vel2)[2] = nullptr,
                                                int (*s_obj)[NODES_G2P] = nullptr) {
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
@@ -598,25 +607,22 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         for (int ii2 = 0; ii2 < 3; ii2++) {
           const int n = row + ii2;
           const float wx = W[0].w[ii2], gx = W[0].g[ii2];
+          float4 q0 = s_vel[n][0], q1 = s_vel[n][1];
           if (MB) {
             const int ob = tag_obj(tag);
             const int sl = (s_obj[0][n] == ob) ? 0 : (s_obj[1][n] == ob) ? 1 : 2;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-              const float vn = sl == 0 ? s_vel[c][n] : sl == 1 ? s_vel2[c][n] : 0.0f;
-              const float an = sl == 0 ? s_vel[3 + c][n] : sl == 1 ? s_vel2[3 + c][n] : 0.0f;
-              S0[c] += wx * vn;
-              S1[c] += gx * vn;
-              Sa[c] += wx * an;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sl != 0) {
+              q0 = (sl == 1) ? s_vel2[n][0] : z4;
+              q1 = (sl == 1) ? s_vel2[n][1] : z4;
             }
-          } else {
+          }
+          const float vn[3] = {q0.x, q0.y, q0.z}, an[3] = {q0.w, q1.x, q1.y};
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-              const float vn = s_vel[c][n];
-              S0[c] += wx * vn;
-              S1[c] += gx * vn;
-              Sa[c] += wx * s_vel[3 + c][n];
-            }
+          for (int c = 0; c < 3; c++) {
+            S0[c] += wx * vn[c];
+            S1[c] += gx * vn[c];
+            Sa[c] += wx * an[c];
           }
         }
 #pragma unroll
@@ -742,13 +748,10 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
-#pragma unroll
-  for (int a = 0; a < 3; a++) s_rec[a][j] = __int_as_float(q[a]);
-  s_rec[3][j] = mp;
-#pragma unroll
-  for (int a = 0; a < 3; a++) s_rec[4 + a][j] = mp * v[a];
-#pragma unroll
-  for (int a = 0; a < 6; a++) s_rec[7 + a][j] = nV * s[a];
+  s_rec->w[j][0] = make_float4(__int_as_float(q[0]), __int_as_float(q[1]), __int_as_float(q[2]), mp);
+  s_rec->w[j][1] = make_float4(mp * v[0], mp * v[1], mp * v[2], nV * s[0]);
+  s_rec->w[j][2] = make_float4(nV * s[1], nV * s[2], nV * s[3], nV * s[4]);
+  s_rec->t[j] = nV * s[5];
   return knew;
 }
 
@@ -758,19 +761,20 @@ struct ScatterRec {
   Axis W[3];
   P2GTerms T;
 };
-__device__ __forceinline__ void load_rec(const InstDesc &I, const float (*s_rec)[CHUNK], int j, ScatterRec &R) {
+__device__ __forceinline__ void load_rec(const InstDesc &I, const RecLds *s_rec, int j, ScatterRec &R) {
+  const float4 w0 = s_rec->w[j][0], w1 = s_rec->w[j][1], w2 = s_rec->w[j][2];
+  const float qf[3] = {w0.x, w0.y, w0.z};
   bool bad = false;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const int q = __float_as_int(s_rec[a][j]);
+    const int q = __float_as_int(qf[a]);
     R.b[a] = stencil_base(q, I.lpq, I.fbits, I.n[a] - 3, bad);
     R.W[a] = gimp3((float)(q - (R.b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
   }
-  R.T.m = s_rec[3][j];
-#pragma unroll
-  for (int a = 0; a < 3; a++) R.T.mv[a] = s_rec[4 + a][j];
-#pragma unroll
-  for (int a = 0; a < 6; a++) R.T.vs[a] = s_rec[7 + a][j];
+  R.T.m = w0.w;
+  R.T.mv[0] = w1.x; R.T.mv[1] = w1.y; R.T.mv[2] = w1.z;
+  R.T.vs[0] = w1.w; R.T.vs[1] = w2.x; R.T.vs[2] = w2.y; R.T.vs[3] = w2.z; R.T.vs[4] = w2.w;
+  R.T.vs[5] = s_rec->t[j];
 }
 
 __device__ __forceinline__ bool region_offset(const ScatterRec &R, const int o[3], int r[3]) {
@@ -824,7 +828,7 @@ __device__ __forceinline__ void scatter_one(const DevPtrs &d, int inst, const Sc
 // Phase B for one list item: record j alone, or records j and j+1 (two particles of one
 // cell) when `two`.
 __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
-                                                 const float (*s_rec)[CHUNK], int j, bool two,
+                                                 const RecLds *s_rec, int j, bool two,
                                                  double (*s_acc)[ACC_N]) {
   ScatterRec P, Q;
   load_rec(I, s_rec, j, P);
@@ -861,9 +865,9 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 template <bool UPDATE, int MASK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_g2p2g(DevPtrs d) {
   static_assert(WG == 4 * TILE3, "four waves per tile");
-  __shared__ float s_vel[6][NODES_G2P];
+  __shared__ float4 s_vel[NODES_G2P][2];
   __shared__ __align__(16) double s_acc[7][ACC_N];
-  __shared__ float s_rec[NREC][CHUNK];
+  __shared__ RecLds s_rec;
   __shared__ unsigned short s_list[CHUNK];
   __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
@@ -908,8 +912,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
           v0 = vp[0];
           v1 = vp[1];
         }
-        s_vel[0][n] = v0.x; s_vel[1][n] = v0.y; s_vel[2][n] = v0.z;
-        s_vel[3][n] = v0.w; s_vel[4][n] = v1.x; s_vel[5][n] = v1.y;
+        s_vel[n][0] = v0;
+        s_vel[n][1] = v1;
       }
     }
     __syncthreads();
@@ -943,7 +947,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       // phase A
       const int i = c0 + tid;
       int knew = -1;
-      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, s_rec, tid);
+      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, tid);
       if (UPDATE) {
         const int r = wave_hist_rank(d.count, knew);
         if (knew >= 0) d.rank[i] = r;
@@ -952,7 +956,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       // phase B
       if (tid < __builtin_amdgcn_readfirstlane(s_nitems)) {
         const int it = s_list[tid];
-        particle_scatter(d, I, o, ii, s_rec, it & 0x7fff, (it & 0x8000) != 0, s_acc);
+        particle_scatter(d, I, o, ii, &s_rec, it & 0x7fff, (it & 0x8000) != 0, s_acc);
       }
       __syncthreads();
     }
@@ -1311,7 +1315,7 @@ __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
 // persistent over the active tiles (as k_g2p2g, without the P2G phase).
 template <int MASK>
 __global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
-  __shared__ float s_vel[6][NODES_G2P], s_vel2[6][NODES_G2P];
+  __shared__ float4 s_vel[NODES_G2P][2], s_vel2[NODES_G2P][2];
   __shared__ int s_obj[2][NODES_G2P];
   __shared__ MatDev s_mat[MAXMAT];
   __shared__ int s_off[TILE3 + 1];
@@ -1349,10 +1353,10 @@ __global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
           cl = d.claim[ni];
         }
       }
-      for (int f = 0; f < 6; f++) {
-        s_vel[f][n] = vq[f];
-        s_vel2[f][n] = vq[6 + f];
-      }
+      s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+      s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
+      s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
+      s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
       s_obj[0][n] = cl.x;
       s_obj[1][n] = cl.y;
     }
@@ -1444,8 +1448,8 @@ void launch_keys(const DevPtrs &d, void *st) {
 void launch_hist(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
-void launch_scan_partial(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_scan_partial, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d);
+void launch_scan_partial(const DevPtrs &d, void *st, int advance) {
+  hipLaunchKernelGGL(k_scan_partial, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d, advance);
 }
 void launch_scan_final(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_scan_final, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d);
@@ -1470,9 +1474,6 @@ void launch_grid(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
 }
 void launch_g2p2g(const DevPtrs &d, void *st) { launch_fused<true>(d, S(st)); }
-void launch_advance(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_advance, dim3(blocks(d.ninst, WG)), dim3(WG), 0, S(st), d);
-}
 void launch_audit(const DevPtrs &d, void *st) {
   const int nb = blocks(d.np, WG);
   hipLaunchKernelGGL(k_audit, dim3(nb), dim3(WG), 0, S(st), d);
