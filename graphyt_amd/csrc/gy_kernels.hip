@@ -499,23 +499,33 @@ constexpr int KM_ALL = 63;
 
 // The fused kernel works through a tile's entries in chunks of CHUNK list entries.
 constexpr int CHUNK = WG;
-// FP64 LDS accumulator region: P2G_R^3 nodes, x fastest, z stride padded to 68 doubles
-// (68 = 4 mod 32) so the 64 lanes of one atomic, spread over different cells, fall on
-// evenly used banks.
+// FP64 LDS accumulator region: P2G_R^3 nodes, x fastest, unpadded, with the x index of
+// rows y = 2, 3 (mod 4) XOR-ed with 4. ds_add_f64 serves 16 lanes at a time over 32
+// four-byte banks, so a double's bank pair is its index mod 16: (x ^ 4*bit1(y)) + 8*(y
+// mod 2) for this layout. Over any four consecutive x and four consecutive y -- the 16
+// cells of one z-layer of a tile, at any stencil offset -- these are 16 distinct pairs,
+// so any 16 consecutive cells of a tile (the lanes of one pass in rank-major order) add
+// without bank conflicts. Without the swizzle rows y and y+2 share banks (two-way
+// conflicts on every pass, measured).
 constexpr int ACC_YS = P2G_R;
-constexpr int ACC_ZS = P2G_R * P2G_R + 4;
+constexpr int ACC_ZS = P2G_R * P2G_R;
 constexpr int ACC_N = ACC_ZS * P2G_R;
-// Phase B scatters particles one per lane (false) or two of one cell per lane (true:
-// half the LDS atomics, twice the weights per lane; measured slower on MI355X).
-constexpr bool P2G_PAIRS = false;
+__device__ __forceinline__ int acc_index(int x, int y, int z) {
+  return (x ^ (((y >> 1) & 1) << 2)) + ACC_YS * y + ACC_ZS * z;
+}
 // P2G record of one particle, handed from the update phase to the scatter phase in LDS:
 // fixed-point position (3), m, m*v (3), -V*sigma (6).
 constexpr int NREC = 13;
 // Records in LDS: three 16-byte words (q0 q1 q2 m | mv0 mv1 mv2 vs0 | vs1 vs2 vs3 vs4) and
-// one more float (vs5) per record.
+// one more float (vs5) per record, with one word (one float) of padding after every 16
+// columns. Phase A writes the particles of one cell to columns 16 apart: unpadded, those
+// are 768 bytes apart, a multiple of the 256-byte LDS bank period (a 16-way conflict for
+// a pass of one cell's lanes); padded, 784 bytes, which spreads them over all banks.
 struct RecLds {
-  float4 w[CHUNK][3];
-  float t[CHUNK];
+  float4 w[3 * CHUNK + CHUNK / 16];
+  float t[CHUNK + CHUNK / 16];
+  __device__ __forceinline__ static int wi(int col) { return 3 * col + (col >> 4); }
+  __device__ __forceinline__ static int ti(int col) { return col + (col >> 4); }
 };
 
 // Rank of a lane's key among the equal keys of its wave, with one global atomic per run
@@ -748,10 +758,11 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
-  s_rec->w[j][0] = make_float4(__int_as_float(q[0]), __int_as_float(q[1]), __int_as_float(q[2]), mp);
-  s_rec->w[j][1] = make_float4(mp * v[0], mp * v[1], mp * v[2], nV * s[0]);
-  s_rec->w[j][2] = make_float4(nV * s[1], nV * s[2], nV * s[3], nV * s[4]);
-  s_rec->t[j] = nV * s[5];
+  float4 *w = &s_rec->w[RecLds::wi(j)];
+  w[0] = make_float4(__int_as_float(q[0]), __int_as_float(q[1]), __int_as_float(q[2]), mp);
+  w[1] = make_float4(mp * v[0], mp * v[1], mp * v[2], nV * s[0]);
+  w[2] = make_float4(nV * s[1], nV * s[2], nV * s[3], nV * s[4]);
+  s_rec->t[RecLds::ti(j)] = nV * s[5];
   return knew;
 }
 
@@ -762,7 +773,8 @@ struct ScatterRec {
   P2GTerms T;
 };
 __device__ __forceinline__ void load_rec(const InstDesc &I, const RecLds *s_rec, int j, ScatterRec &R) {
-  const float4 w0 = s_rec->w[j][0], w1 = s_rec->w[j][1], w2 = s_rec->w[j][2];
+  const float4 *w = &s_rec->w[RecLds::wi(j)];
+  const float4 w0 = w[0], w1 = w[1], w2 = w[2];
   const float qf[3] = {w0.x, w0.y, w0.z};
   bool bad = false;
 #pragma unroll
@@ -774,7 +786,7 @@ __device__ __forceinline__ void load_rec(const InstDesc &I, const RecLds *s_rec,
   R.T.m = w0.w;
   R.T.mv[0] = w1.x; R.T.mv[1] = w1.y; R.T.mv[2] = w1.z;
   R.T.vs[0] = w1.w; R.T.vs[1] = w2.x; R.T.vs[2] = w2.y; R.T.vs[3] = w2.z; R.T.vs[4] = w2.w;
-  R.T.vs[5] = s_rec->t[j];
+  R.T.vs[5] = s_rec->t[RecLds::ti(j)];
 }
 
 __device__ __forceinline__ bool region_offset(const ScatterRec &R, const int o[3], int r[3]) {
@@ -783,66 +795,37 @@ __device__ __forceinline__ bool region_offset(const ScatterRec &R, const int o[3
   return r[0] >= 0 && r[1] >= 0 && r[2] >= 0 && r[0] + 2 < P2G_R && r[1] + 2 < P2G_R && r[2] + 2 < P2G_R;
 }
 
-// 27-node scatter of m, m*v and -V*sigma.gradN (Extrapolation.cpp:166-286) of one or two
-// particles with the same stencil base into the tile's FP64 LDS region: the two
-// particles' FP32 terms of a node are added before the one FP64 atomic per field.
-template <bool TWO>
-__device__ __forceinline__ void scatter_lds(const ScatterRec &P, const ScatterRec &Q, const int r[3],
-                                            double (*s_acc)[ACC_N]) {
+// 27-node scatter of m, m*v and -V*sigma.gradN (Extrapolation.cpp:166-286) of one
+// particle into the tile's FP64 LDS region.
+__device__ __forceinline__ void scatter_lds(const ScatterRec &P, const int r[3], double (*s_acc)[ACC_N]) {
 #pragma unroll 1
   for (int kk = 0; kk < 3; kk++) {
     const float wz = sel3(kk, P.W[2].w[0], P.W[2].w[1], P.W[2].w[2]);
     const float gz = sel3(kk, P.W[2].g[0], P.W[2].g[1], P.W[2].g[2]);
-    const float wz2 = TWO ? sel3(kk, Q.W[2].w[0], Q.W[2].w[1], Q.W[2].w[2]) : 0.0f;
-    const float gz2 = TWO ? sel3(kk, Q.W[2].g[0], Q.W[2].g[1], Q.W[2].g[2]) : 0.0f;
 #pragma unroll
     for (int jj = 0; jj < 3; jj++) {
       const float Aw = P.W[1].w[jj] * wz, By = P.W[1].g[jj] * wz, Bz = P.W[1].w[jj] * gz;
-      const float Aw2 = TWO ? Q.W[1].w[jj] * wz2 : 0.0f, By2 = TWO ? Q.W[1].g[jj] * wz2 : 0.0f,
-                  Bz2 = TWO ? Q.W[1].w[jj] * gz2 : 0.0f;
-      const int row = r[0] + ACC_YS * (r[1] + jj) + ACC_ZS * (r[2] + kk);
 #pragma unroll
       for (int ii2 = 0; ii2 < 3; ii2++) {
+        const int a = acc_index(r[0] + ii2, r[1] + jj, r[2] + kk);
         float tt[7];
         node_terms(P.T, P.W[0].w[ii2] * Aw, P.W[0].g[ii2] * Aw, P.W[0].w[ii2] * By, P.W[0].w[ii2] * Bz, tt);
-        if (TWO) {
-          float t2[7];
-          node_terms(Q.T, Q.W[0].w[ii2] * Aw2, Q.W[0].g[ii2] * Aw2, Q.W[0].w[ii2] * By2, Q.W[0].w[ii2] * Bz2, t2);
 #pragma unroll
-          for (int f = 0; f < 7; f++) tt[f] += t2[f];
-        }
-#pragma unroll
-        for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][row + ii2], (double)tt[f]);
+        for (int f = 0; f < 7; f++) atomicAdd(&s_acc[f][a], (double)tt[f]);
       }
     }
   }
 }
 
-__device__ __forceinline__ void scatter_one(const DevPtrs &d, int inst, const ScatterRec &P, const int o[3],
-                                            double (*s_acc)[ACC_N]) {
-  int r[3];
-  if (region_offset(P, o, r)) scatter_lds<false>(P, P, r, s_acc);
-  else p2g_global(d.acc, d.brick_flag, &d.inst[inst], P.b[0], P.b[1], P.b[2], P.W[0], P.W[1], P.W[2], P.T);
-}
-
-// Phase B for one list item: record j alone, or records j and j+1 (two particles of one
-// cell) when `two`.
+// Phase B for the record in column j: into the LDS region, or (a particle whose new
+// stencil left the region) by global atomics.
 __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDesc &I, const int o[3], int inst,
-                                                 const RecLds *s_rec, int j, bool two,
-                                                 double (*s_acc)[ACC_N]) {
-  ScatterRec P, Q;
+                                                 const RecLds *s_rec, int j, double (*s_acc)[ACC_N]) {
+  ScatterRec P;
   load_rec(I, s_rec, j, P);
-  if (P2G_PAIRS && two) {
-    load_rec(I, s_rec, j + 1, Q);
-    int r[3];
-    const bool same = P.b[0] == Q.b[0] && P.b[1] == Q.b[1] && P.b[2] == Q.b[2];
-    if (same && region_offset(P, o, r)) {
-      scatter_lds<true>(P, Q, r, s_acc);
-      return;
-    }
-    scatter_one(d, inst, Q, o, s_acc);  // the pair's stencils differ (a particle changed cell)
-  }
-  scatter_one(d, inst, P, o, s_acc);
+  int r[3];
+  if (region_offset(P, o, r)) scatter_lds(P, r, s_acc);
+  else p2g_global(d.acc, d.brick_flag, &d.inst[inst], P.b[0], P.b[1], P.b[2], P.W[0], P.W[1], P.W[2], P.T);
 }
 
 // The fused particle kernel, persistent over the active tile list. UPDATE = false: P2G of
@@ -854,11 +837,16 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 //     node velocities, update, constitutive/damage models, store at the entry's sorted
 //     position, new key with one histogram atomic per run of equal keys, and the P2G
 //     record into LDS;
-//  B. lane = pair of entries of one cell, pairs in rank-major order (the r-th pairs of
-//     every cell first), so the lanes of one LDS atomic belong to different cells and hit
-//     different nodes: 27-node scatter of both particles into the FP64 LDS region
-//     [o-1, o+TILE+3)^3, one atomic per node and field for the pair.
-// ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32 LDS atomic adds. After the last
+//  B. lane = record, 27-node scatter into the FP64 LDS region [o-1, o+TILE+3)^3.
+// ds_add_f64 is a fast LDS atomic on gfx950, unlike FP32 LDS atomic adds; it serves 16
+// lanes per pass, and two lanes of a pass conflict when their nodes share a bank pair.
+// With the region layout of acc_index, cells c and c' share bank pairs (at every stencil
+// offset) iff c = c' mod 16, their "class". Phase A therefore writes each record to the
+// column that a class-sorted, transposed order gives it: the chunk's entries sorted by
+// class (whole cells, so the entries of one class are consecutive: position j), and
+// j -> column 16*(j mod P) + j/P for P = ceil(n/16) passes. The entries of a class, the
+// particles of one cell among them, land in different passes unless the class holds
+// more than P entries of the chunk. After the last
 // chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
 // is stamped; the grid kernel gathers the partials of the tiles covering each node in a
 // fixed order.
@@ -868,10 +856,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ float4 s_vel[NODES_G2P][2];
   __shared__ __align__(16) double s_acc[7][ACC_N];
   __shared__ RecLds s_rec;
-  __shared__ unsigned short s_list[CHUNK];
   __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
-  __shared__ int s_tile, s_nitems;
+  __shared__ int s_tile;
   const int tid = threadIdx.x;
   const int grp = blockIdx.x & 7;  // dispatch XCD of this workgroup (round-robin)
   const int nact = d.ctrl->active_tiles;
@@ -922,42 +909,48 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
     for (int c0 = p0; c0 < p1; c0 += CHUNK) {
       const int c1 = min(c0 + CHUNK, p1), nc = c1 - c0;
-      // rank-major order of this chunk's entries, or of pairs of one cell's entries (wave
-      // 0, lane = cell of the tile): item = first entry | (second entry present) << 15
-      if (tid < TILE3) {
-        const int elo = max(s_off[tid], c0), ehi = min(s_off[tid + 1], c1);
-        const int n = max(ehi - elo, 0);
-        const int np2 = P2G_PAIRS ? (n + 1) >> 1 : n;
-        int maxn = np2;
-#pragma unroll
-        for (int w = 32; w > 0; w >>= 1) maxn = max(maxn, __shfl_xor(maxn, w, 64));
-        const unsigned long long below = (1ull << tid) - 1ull;
-        int S = 0;
-        for (int r = 0; r < maxn; r++) {
-          const unsigned long long m = __ballot(np2 > r);
-          if (np2 > r) {
-            const int e = elo - c0 + (P2G_PAIRS ? 2 * r : r);
-            s_list[S + __popcll(m & below)] =
-                (unsigned short)(e | ((P2G_PAIRS && 2 * r + 1 < n) ? 0x8000 : 0));
-          }
-          S += __popcll(m);
-        }
-        if (tid == 0) s_nitems = S;
-      }
-      // phase A
+      // cell of this lane's entry: the last cell whose first entry is <= i (two rounds of
+      // LDS reads over the tile's 65 offsets)
       const int i = c0 + tid;
+      const int lane = tid & 63;
+      int ce = 0;
+      {
+        int k = 0;
+#pragma unroll
+        for (int m = 1; m < 8; m++) k += (s_off[8 * m] <= i) ? 1 : 0;
+        int c = 8 * k;
+#pragma unroll
+        for (int m = 1; m < 8; m++) c += (s_off[8 * k + m] <= i) ? 1 : 0;
+        ce = c;
+      }
+      // class-sorted position j of the entry: the chunk's cells in class order (c mod 16,
+      // then c / 16), an exclusive scan of their in-chunk counts over the wave (lane = cell)
+      const int clo = max(s_off[lane], c0), chi = min(s_off[lane + 1], c1);
+      const int ncell = max(chi - clo, 0);
+      const int nL = __shfl(ncell, 16 * (lane & 3) + (lane >> 2), 64);  // lane L: cell in class order
+      int incl = nL;
+#pragma unroll
+      for (int w = 1; w < 64; w <<= 1) {
+        const int u = __shfl_up(incl, w, 64);
+        if (lane >= w) incl += u;
+      }
+      const int cpos = __shfl(incl - nL, 4 * (ce & 15) + (ce >> 4), 64);
+      const int cfirst = __shfl(clo, ce, 64);
+      const int P = (nc + 15) >> 4;
+      const float invP = 1.0f / (float)P;
+      const int j = cpos + (i - cfirst);
+      const int jq = (int)(((float)j + 0.5f) * invP);  // j / P (exact: j < 256, P <= 16)
+      const int col = 16 * (j - jq * P) + jq;
+      // phase A
       int knew = -1;
-      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, tid);
+      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col);
       if (UPDATE) {
         const int r = wave_hist_rank(d.count, knew);
         if (knew >= 0) d.rank[i] = r;
       }
       __syncthreads();
-      // phase B
-      if (tid < __builtin_amdgcn_readfirstlane(s_nitems)) {
-        const int it = s_list[tid];
-        particle_scatter(d, I, o, ii, &s_rec, it & 0x7fff, (it & 0x8000) != 0, s_acc);
-      }
+      // phase B: column tid holds class-sorted entry (tid % 16) * P + tid / 16
+      if ((tid >> 4) < P && (tid & 15) * P + (tid >> 4) < nc) particle_scatter(d, I, o, ii, &s_rec, tid, s_acc);
       __syncthreads();
     }
 
@@ -967,7 +960,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     bool halo = false;
     for (int n = tid; n < P2G_NODES; n += WG) {
       const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
-      const int a = x + ACC_YS * y + ACC_ZS * z;
+      const int a = acc_index(x, y, z);
       const bool shell = x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R;
       if (!shell) {
         const int c = (x - 1) + CORE_R * ((y - 1) + CORE_R * (z - 1));
@@ -987,7 +980,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       for (int n = tid; n < P2G_NODES; n += WG) {
         const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
         if (!(x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R)) continue;
-        const int a = x + ACC_YS * y + ACC_ZS * z;
+        const int a = acc_index(x, y, z);
 #pragma unroll
         for (int f = 0; f < 7; f++) pt[f * P2G_NODES + n] = (float)s_acc[f][a];
       }
