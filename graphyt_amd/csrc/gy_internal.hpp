@@ -114,7 +114,12 @@ struct Ctrl {
   int oor;           // count of out-of-range particles (clamped)
   int epoch;         // tile partials written with tile_epoch == epoch are this step's P2G
   int active_tiles;  // tiles holding particles in the current binning
-  int next[8];       // per-XCD work counters of the persistent particle kernel
+  int xlo[9];        // per-XCD ranges [xlo[g], xlo[g+1]) of the active tiles in tile order
+  int hxlo[9];       // heavy tiles (more than one chunk) before xlo[g], in tile order
+  // per-XCD work counters of the persistent particle kernels, on a line of their own:
+  // every workgroup adds to them, and loads of fields sharing their line stall behind
+  // those atomics
+  alignas(256) int next[8];
 };
 
 // All device pointers of a context (passed by value to kernels).
@@ -127,7 +132,7 @@ struct DevPtrs {
   int *offsets;          // per-key exclusive offsets (nkeys + 1)
   int *blocksum;         // scan partials (particle counts per scan block)
   int *blocktiles;       // scan partials (active tiles per scan block)
-  int *active;           // active tile list, ascending tile id
+  int *active;           // active tiles: heavy ones ascending from the front, light ones from the back
   int *tile_epoch;       // per tile: epoch of its last written core partials
   int *halo_epoch;       // per tile: epoch of its last written halo-shell partials
   float *pcore;          // per tile: P2G sums of the core [o, o+TILE+2)^3, [7][CORE_R^3]

@@ -32,6 +32,7 @@ constexpr int WG = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
 constexpr int MAXMAT = 16;
+constexpr int CHUNK_ENTRIES = WG;  // list entries per chunk of the fused particle kernel
 constexpr int NODES_G2P = G2P_R * G2P_R * G2P_R;
 constexpr int NODES_P2G = P2G_NODES;
 
@@ -227,21 +228,37 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
   const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
   const int tot = block_sum(s);
   const int tact = block_sum(act);
+  const int theavy = block_sum((act && ts > CHUNK_ENTRIES) ? 1 : 0);
   if (threadIdx.x == 0) {
     d.blocksum[blockIdx.x] = tot;
-    d.blocktiles[blockIdx.x] = tact;
+    d.blocktiles[blockIdx.x] = tact | (theavy << 16);  // at most SCAN_BLOCK / TILE3 each
   }
 }
 
+// A tile's fixed cost in the particle kernel (set-up, flush), in particle equivalents:
+// the weight by which the active tile list is cut into equal per-XCD shares.
+constexpr int TILE_COST = 192;
+
 __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
-  // prefixes of all earlier blocks (particles and active tiles)
-  int pre = 0, pret = 0;
-  for (int j = threadIdx.x; j < (int)blockIdx.x; j += WG) {
-    pre += d.blocksum[j];
-    pret += d.blocktiles[j];
+  // prefixes of all earlier blocks (particles and active tiles), and the totals
+  int pre = 0, pret = 0, preh = 0, all = 0, allt = 0, allh = 0;
+  for (int j = threadIdx.x; j < (int)gridDim.x; j += WG) {
+    const int bs = d.blocksum[j], bt = d.blocktiles[j] & 0xffff, bh = d.blocktiles[j] >> 16;
+    if (j < (int)blockIdx.x) {
+      pre += bs;
+      pret += bt;
+      preh += bh;
+    }
+    all += bs;
+    allt += bt;
+    allh += bh;
   }
   pre = block_sum(pre);
   pret = block_sum(pret);
+  preh = block_sum(preh);
+  all = block_sum(all);
+  allt = block_sum(allt);
+  allh = block_sum(allh);
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   const bool have = base < d.nkeys;
   int c[SCAN_ITEMS];
@@ -254,6 +271,7 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   }
   int tot;
   int run = pre + block_exclusive_scan(s, tot);
+  const int run0 = run;  // first entry of this thread's keys
   if (have) {
     int4 *ov = reinterpret_cast<int4 *>(d.offsets + base);
     int4 *cv = reinterpret_cast<int4 *>(d.count + base);
@@ -268,13 +286,37 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
       cv[q] = make_int4(0, 0, 0, 0);
     }
   }
-  // compacted active tile list in ascending tile order
+  // compacted active tile list: heavy tiles (more than one chunk of entries) ascending
+  // from the front, light ones ascending from the back
   int ts = s + __shfl_xor(s, 1, 64);
   ts += __shfl_xor(ts, 2, 64);
   const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
-  int ttot;
-  const int pos = pret + block_exclusive_scan(act, ttot);
-  if (act) d.active[pos] = (int)(base / TILE3);
+  const int heavy = (act && ts > CHUNK_ENTRIES) ? 1 : 0;
+  int ttot, htot;
+  const int pos = pret + block_exclusive_scan(act, ttot);  // index in tile order
+  const int hpos = preh + block_exclusive_scan(heavy, htot);  // heavy tiles before it
+  if (act) d.active[heavy ? hpos : allt - 1 - (pos - hpos)] = (int)(base / TILE3);
+  // per-XCD ranges of the active list for the particle kernel: equal shares of the
+  // weight (particles + TILE_COST per tile), cut at tile boundaries. Active tile `pos`
+  // spans weights [w0, w1); the tile holding g/8 of the total starts range g.
+  if (act) {
+    const long long wt = (long long)all + (long long)TILE_COST * allt;
+    const long long w0 = (long long)run0 + (long long)TILE_COST * pos, w1 = w0 + ts + TILE_COST;
+    for (int g = 1; g < 8; g++) {
+      const long long target = g * wt / 8;
+      if (w0 <= target && target < w1) {
+        d.ctrl->xlo[g] = pos;
+        d.ctrl->hxlo[g] = hpos;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d.ctrl->xlo[0] = d.ctrl->hxlo[0] = 0;
+    d.ctrl->xlo[8] = allt;
+    d.ctrl->hxlo[8] = allh;
+    if (allt == 0)
+      for (int g = 1; g < 8; g++) d.ctrl->xlo[g] = d.ctrl->hxlo[g] = 0;
+  }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     d.offsets[d.nkeys] = pre + tot;
     d.ctrl->active_tiles = pret + ttot;
@@ -498,7 +540,7 @@ constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 
 constexpr int KM_ALL = 63;
 
 // The fused kernel works through a tile's entries in chunks of CHUNK list entries.
-constexpr int CHUNK = WG;
+constexpr int CHUNK = CHUNK_ENTRIES;
 // FP64 LDS accumulator region: P2G_R^3 nodes, x fastest, unpadded, with the x index of
 // rows y = 2, 3 (mod 4) XOR-ed with 4. ds_add_f64 serves 16 lanes at a time over 32
 // four-byte banks, so a double's bank pair is its index mod 16: (x ^ 4*bit1(y)) + 8*(y
@@ -549,6 +591,31 @@ __device__ __forceinline__ int wave_hist_rank(int *count, int key) {
   const int hl = mine ? 63 - __clzll((long long)mine) : lane;
   base = __shfl(base, hl, 64);
   return base + lane - hl;
+}
+
+// wave_hist_rank in two halves, so the atomic's round trip can overlap other work:
+// hist_issue adds the run lengths (head lanes get the old counts in `base`), hist_rank
+// then spreads each run's base over its lanes.
+__device__ __forceinline__ int hist_issue(int *count, int key, int &hl) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = key >= 0 && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valid = __ballot(key >= 0);
+  const unsigned long long upto = (2ull << lane) - 1ull;
+  int base = 0;
+  if (head) {
+    const unsigned long long later = heads & ~upto;
+    const int vend = 64 - __clzll((long long)valid);
+    const int end = later ? __ffsll((long long)later) - 1 : vend;
+    base = atomicAdd(&count[key], end - lane);
+  }
+  const unsigned long long mine = heads & upto;
+  hl = mine ? 63 - __clzll((long long)mine) : lane;
+  return base;
+}
+__device__ __forceinline__ int hist_rank(int base, int hl) {
+  return __shfl(base, hl, 64) + (int)(threadIdx.x & 63) - hl;
 }
 
 // Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
@@ -850,6 +917,26 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 // chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
 // is stamped; the grid kernel gathers the partials of the tiles covering each node in a
 // fixed order.
+// The active tiles of XCD group g: its heavy tiles first (in tile order), then its light
+// ones, so the claims that end the launch are the short ones. Read once per launch.
+struct TileQueue {
+  int h0, nh, l0, n, nact;
+  __device__ __forceinline__ TileQueue(const DevPtrs &d, int g) {
+    const int x0 = d.ctrl->xlo[g], x1 = d.ctrl->xlo[g + 1];
+    h0 = d.ctrl->hxlo[g];
+    nh = d.ctrl->hxlo[g + 1] - h0;
+    l0 = x0 - h0;  // light tiles before the range
+    n = x1 - x0;
+    nact = d.ctrl->active_tiles;
+  }
+  // tile of claim k, -1 past the end
+  __device__ __forceinline__ int tile(const DevPtrs &d, int k) const {
+    if (k < nh) return d.active[h0 + k];
+    if (k < n) return d.active[nact - 1 - (l0 + k - nh)];
+    return -1;
+  }
+};
+
 template <bool UPDATE, int MASK>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_g2p2g(DevPtrs d) {
   static_assert(WG == 4 * TILE3, "four waves per tile");
@@ -861,15 +948,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ int s_tile;
   const int tid = threadIdx.x;
   const int grp = blockIdx.x & 7;  // dispatch XCD of this workgroup (round-robin)
-  const int nact = d.ctrl->active_tiles;
-  const int lo = (int)((long long)grp * nact / 8), hi = (int)((long long)(grp + 1) * nact / 8);
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
+  const TileQueue queue(d, grp);
 
   for (;;) {
     if (tid == 0) {
-      const int jt = lo + atomicAdd(&d.ctrl->next[grp], 1);
-      s_tile = (jt < hi) ? d.active[jt] : -1;
+      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next[grp], 1));
     }
     __syncthreads();
     // workgroup-uniform values read from LDS: keep them (and all that derives from
@@ -944,13 +1029,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       // phase A
       int knew = -1;
       if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col);
-      if (UPDATE) {
-        const int r = wave_hist_rank(d.count, knew);
-        if (knew >= 0) d.rank[i] = r;
-      }
+      int hl = 0, hbase = 0;
+      if (UPDATE) hbase = hist_issue(d.count, knew, hl);
       __syncthreads();
       // phase B: column tid holds class-sorted entry (tid % 16) * P + tid / 16
       if ((tid >> 4) < P && (tid & 15) * P + (tid >> 4) < nc) particle_scatter(d, I, o, ii, &s_rec, tid, s_acc);
+      if (UPDATE) {
+        const int r = hist_rank(hbase, hl);
+        if (knew >= 0) d.rank[i] = r;
+      }
       __syncthreads();
     }
 
@@ -1315,13 +1402,11 @@ __global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
   __shared__ int s_tile;
   const int tid = threadIdx.x;
   const int grp = blockIdx.x & 7;
-  const int nact = d.ctrl->active_tiles;
-  const int lo = (int)((long long)grp * nact / 8), hi = (int)((long long)(grp + 1) * nact / 8);
   const int par = d.ctrl->parity;
+  const TileQueue queue(d, grp);
   for (;;) {
     if (tid == 0) {
-      const int jt = lo + atomicAdd(&d.ctrl->next[grp], 1);
-      s_tile = (jt < hi) ? d.active[jt] : -1;
+      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next[grp], 1));
     }
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_tile);
