@@ -273,6 +273,7 @@ int alloc_grid(gy_ctx *c) {
   if ((rc = dalloc(c, &d.bc_type, (size_t)node_base))) return rc;
   if ((rc = dalloc(c, &d.bc_val, (size_t)node_base * 2))) return rc;
   if ((rc = dalloc(c, &d.brick_flag, (size_t)brick_base))) return rc;
+  if ((rc = dalloc(c, &d.brick_list, (size_t)brick_base))) return rc;
   if ((rc = dalloc(c, &d.tile_inst, (size_t)tile_base))) return rc;
   if ((rc = dalloc(c, &d.brick_inst, (size_t)brick_base))) return rc;
   if ((rc = dalloc(c, &d.count, (size_t)d.nkeys))) return rc;
@@ -282,6 +283,7 @@ int alloc_grid(gy_ctx *c) {
   if ((rc = dalloc(c, &d.active, (size_t)tile_base + 1))) return rc;
   if ((rc = dalloc(c, &d.tile_epoch, (size_t)tile_base))) return rc;
   if ((rc = dalloc(c, &d.halo_epoch, (size_t)tile_base))) return rc;
+  if ((rc = dalloc(c, &d.nodemask, (size_t)tile_base * (P2G_NODES / 64)))) return rc;
   if ((rc = dalloc(c, &d.pcore, (size_t)tile_base * CORE_W))) return rc;
   if ((rc = dalloc(c, &d.partial, (size_t)tile_base * PARTIAL_W))) return rc;
   {
@@ -351,6 +353,7 @@ void enqueue_init(gy_ctx *c) {
     return;
   }
   timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
+  launch_brick_list(c->d, s);  // within a step the scan builds it
 }
 
 void drop_graph(gy_ctx *c) {
@@ -954,10 +957,10 @@ void gy_destroy(gy_ctx *c) {
   DevPtrs &d = c->d;
   for (int b = 0; b < 2; b++) { dfree(d.pbuf[b]); dfree(d.key[b]); }
   dfree(d.rank); dfree(d.tmp); dfree(d.count); dfree(d.offsets); dfree(d.blocksum);
-  dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.bc_type); dfree(d.bc_val);
+  dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.brick_list); dfree(d.bc_type); dfree(d.bc_val);
   dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
   dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
-  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.pcore); dfree(d.partial);
+  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.nodemask); dfree(d.pcore); dfree(d.partial);
   dfree(d.acc2); dfree(d.nrm); dfree(d.claim); dfree(d.velb); dfree(d.cellobj);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

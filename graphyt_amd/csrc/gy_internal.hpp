@@ -28,10 +28,10 @@ constexpr int G2P_R = TILE + 2;            // staged VEL region edge (nodes)
 constexpr int P2G_R = TILE + 4;            // LDS accumulator region edge (nodes), halo -1/+1
 constexpr int NODE_W = 8;                  // floats per node record
 constexpr int P2G_NODES = P2G_R * P2G_R * P2G_R;  // nodes of a tile's P2G region
-constexpr int PARTIAL_W = 7 * P2G_NODES;   // floats of one tile's full-region partial sums
+constexpr int PARTIAL_W = 8 * P2G_NODES;   // floats of one tile's full-region partial sums
 constexpr int CORE_R = TILE + 2;           // edge of the region a tile's own cells reach
 constexpr int CORE_NODES = CORE_R * CORE_R * CORE_R;
-constexpr int CORE_W = 7 * CORE_NODES;     // floats of one tile's core partial sums
+constexpr int CORE_W = 8 * CORE_NODES;     // floats of one tile's core partial sums
 
 enum PField : int {
   F_QX = 0, F_QY, F_QZ,                     // fixed-point position (int32 bits)
@@ -120,6 +120,7 @@ struct Ctrl {
   // every workgroup adds to them, and loads of fields sharing their line stall behind
   // those atomics
   alignas(256) int next[8];
+  int active_bricks;  // length of the flagged-brick list (appended with atomics)
 };
 
 // All device pointers of a context (passed by value to kernels).
@@ -135,13 +136,15 @@ struct DevPtrs {
   int *active;           // active tiles: heavy ones ascending from the front, light ones from the back
   int *tile_epoch;       // per tile: epoch of its last written core partials
   int *halo_epoch;       // per tile: epoch of its last written halo-shell partials
-  float *pcore;          // per tile: P2G sums of the core [o, o+TILE+2)^3, [7][CORE_R^3]
+  uint64_t *nodemask;    // per tile: 8 words, bit l set when region node l has mass
+  float *pcore;          // per tile: P2G sums of the core [o, o+TILE+2)^3, [CORE_R^3][8] (m, mv, f, 0)
   float *partial;        // per tile: P2G sums of the LDS region [o-1, o+TILE+3)^3,
-                         // [7][P2G_R^3]; only the shell outside the core is read, and
-                         // only when halo_epoch is current
+                         // [P2G_R^3][8]; only the shell outside the core is written and
+                         // read, the nodes with a bit in nodemask
   float *acc;            // NODE_W per global node: P2G of particles outside their tile's region
   float *vel;            // NODE_W per global node
   uint8_t *brick_flag;   // per global canonical brick
+  int *brick_list;       // flagged bricks (any order), built after the particle kernel
   uint8_t *bc_type;      // per global node
   float *bc_val;         // 2 per global node
   float *pbc_val;        // 4 per particle (indexed by pid)
@@ -195,6 +198,7 @@ void launch_scan_final(const DevPtrs &d, void *stream);
 void launch_place(const DevPtrs &d, void *stream);
 void launch_p2g_only(const DevPtrs &d, void *stream);
 void launch_grid(const DevPtrs &d, void *stream);
+void launch_brick_list(const DevPtrs &d, void *stream);  // flagged-brick list outside the step
 void launch_g2p2g(const DevPtrs &d, void *stream);
 void launch_audit(const DevPtrs &d, void *stream);
 void launch_stable_order(const DevPtrs &d, int *order, void *stream);

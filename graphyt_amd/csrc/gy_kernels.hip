@@ -203,6 +203,24 @@ __device__ __forceinline__ void load16(const int *p, int c[16]) {
 
 // With `advance`, block 0 also performs the step bookkeeping: times, step, parity and
 // epoch (no block of this kernel reads them; the kernels after it see the new values).
+// Append the flagged bricks among gb, gb + stride, ... to the brick list: one atomic per
+// wave and round. All lanes of the wave take part (uniform trip count).
+__device__ __forceinline__ void append_bricks(const DevPtrs &d, int gb, int stride) {
+  const int lane = threadIdx.x & 63;
+  const int wave0 = gb - lane;
+  for (int w = wave0; w < d.nbricks; w += stride) {
+    const int b = w + lane;
+    const bool f = b < d.nbricks && d.brick_flag[b] != 0;
+    const unsigned long long m = __ballot(f);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&d.ctrl->active_bricks, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (f) d.brick_list[base + __popcll(m & ((1ull << lane) - 1ull))] = b;
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
   if (advance && blockIdx.x == 0) {
     for (int i = threadIdx.x; i < d.ninst; i += WG) {
@@ -215,6 +233,8 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
       d.ctrl->epoch += 1;
     }
   }
+  // the brick list is rebuilt by the scan's second kernel
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->active_bricks = 0;
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   if (base < d.nkeys) {
@@ -322,6 +342,14 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     d.ctrl->active_tiles = pret + ttot;
   }
   if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  // list of the bricks the particle kernel flagged, for the next grid update
+  append_bricks(d, blockIdx.x * WG + threadIdx.x, gridDim.x * WG);
+}
+
+// Brick list outside the step sequence (after the first P2G of an upload).
+__global__ void k_brick_list_reset(DevPtrs d) { d.ctrl->active_bricks = 0; }
+__global__ __launch_bounds__(WG) void k_brick_list(DevPtrs d) {
+  append_bricks(d, blockIdx.x * WG + threadIdx.x, gridDim.x * WG);
 }
 
 __global__ __launch_bounds__(WG) void k_place(DevPtrs d) {
@@ -353,61 +381,52 @@ __global__ __launch_bounds__(WG) void k_stable_order(DevPtrs d, int *order) {
 // Sum of the tile partials covering actual node (a0, a1, a2): a node n lies in the P2G
 // region [o-1, o+TILE+3) of the tiles with o = TILE*(tt-1) and TILE*tt, tt = (n+1)/TILE,
 // at region coordinate l = n - o + 1. Coordinates 1..TILE+2 on every axis are the tile's
-// core (read when the tile wrote partials this step, tile_epoch == epoch); the rest is
-// the halo shell, which only particles that changed cell reach (read when halo_epoch ==
-// epoch). Fixed order: deterministic.
+// core partials, the rest its halo shell (written only when a particle that changed cell
+// reached it). A tile contributes when it wrote partials this step (tile_epoch == epoch)
+// and its node mask has the node's bit (the node has mass in that tile's region), so
+// zero partials are never read. Fixed order (z, y, x offset): deterministic.
 __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc &I, int epoch, int a0, int a1,
                                                 int a2, float acc7[7]) {
   const int t0 = (a0 + 1) / TILE, t1 = (a1 + 1) / TILE, t2 = (a2 + 1) / TILE;
+  bool use[8];
+  const float4 *pp[8];
 #pragma unroll
-  for (int dz = 0; dz < 2; dz++) {
-    const int tz = t2 - 1 + dz;
-    if (tz < 0 || tz >= I.nt[2]) continue;
-    const int lz = a2 - TILE * tz + 1;
+  for (int k = 0; k < 8; k++) {
+    const int tx = t0 - 1 + (k & 1), ty = t1 - 1 + ((k >> 1) & 1), tz = t2 - 1 + (k >> 2);
+    const bool in = tx >= 0 && tx < I.nt[0] && ty >= 0 && ty < I.nt[1] && tz >= 0 && tz < I.nt[2];
+    const int lx = a0 - TILE * tx + 1, ly = a1 - TILE * ty + 1, lz = a2 - TILE * tz + 1;  // in [0, P2G_R)
+    const int t = in ? I.tile_base + tx + I.nt[0] * (ty + I.nt[1] * tz) : I.tile_base;
+    const int l = lx + P2G_R * (ly + P2G_R * lz);
+    const uint64_t word = d.nodemask[(long long)t * (P2G_NODES / 64) + (l >> 6)];
+    use[k] = in && d.tile_epoch[t] == epoch && ((word >> (l & 63)) & 1ull);
+    const bool core = lx >= 1 && lx <= CORE_R && ly >= 1 && ly <= CORE_R && lz >= 1 && lz <= CORE_R;
+    const int c = (lx - 1) + CORE_R * ((ly - 1) + CORE_R * (lz - 1));
+    pp[k] = reinterpret_cast<const float4 *>(core ? d.pcore + (long long)t * CORE_W + 8 * c
+                                                  : d.partial + (long long)t * PARTIAL_W + 8 * l);
+  }
+  // a node's partial is (m, mv0, mv1, mv2 | f0, f1, f2, -): two 16-byte loads
+  float4 u[8], w[8];
 #pragma unroll
-    for (int dy = 0; dy < 2; dy++) {
-      const int ty = t1 - 1 + dy;
-      if (ty < 0 || ty >= I.nt[1]) continue;
-      const int ly = a1 - TILE * ty + 1;
+  for (int k = 0; k < 8; k++) {
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    u[k] = use[k] ? pp[k][0] : z;
+    w[k] = use[k] ? pp[k][1] : z;
+  }
 #pragma unroll
-      for (int dx = 0; dx < 2; dx++) {
-        const int tx = t0 - 1 + dx;
-        if (tx < 0 || tx >= I.nt[0]) continue;
-        const int lx = a0 - TILE * tx + 1;
-        const int t = I.tile_base + tx + I.nt[0] * (ty + I.nt[1] * tz);
-        const bool core = lx >= 1 && lx <= CORE_R && ly >= 1 && ly <= CORE_R && lz >= 1 && lz <= CORE_R;
-        const float *pp;
-        int stride;
-        if (core) {
-          if (d.tile_epoch[t] != epoch) continue;
-          pp = d.pcore + (long long)t * CORE_W + (lx - 1) + CORE_R * ((ly - 1) + CORE_R * (lz - 1));
-          stride = CORE_NODES;
-        } else {
-          if (d.halo_epoch[t] != epoch) continue;
-          pp = d.partial + (long long)t * PARTIAL_W + lx + P2G_R * (ly + P2G_R * lz);
-          stride = P2G_NODES;
-        }
+  for (int k = 0; k < 8; k++) {
+    const float v[7] = {u[k].x, u[k].y, u[k].z, u[k].w, w[k].x, w[k].y, w[k].z};
 #pragma unroll
-        for (int f = 0; f < 7; f++) acc7[f] += pp[f * stride];
-      }
-    }
+    for (int f = 0; f < 7; f++) acc7[f] = use[k] ? acc7[f] + v[f] : acc7[f];
   }
 }
 
-// Grid update on the active canonical bricks (one wave per 4x4x4 brick): gather the P2G
+// Grid update of one flagged canonical brick (one wave per 4x4x4 brick): gather the P2G
 // partials of every actual node of the periodic class, then solve.
-__global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
-  // the particle kernel that follows walks the active tile list from the start
-  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gb = blockIdx.x * (WG / 64) + wave;
-  if (gb >= d.nbricks) return;
-  if (!d.brick_flag[gb]) return;
+__device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
   const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
-  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
   if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
     // P2G sums over the periodic class (Grid.cpp:320-602: node i and node N-3+i share sums)
     float a7[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -477,6 +496,16 @@ __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
     }
   }
   if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// Grid update, persistent over the list of flagged bricks.
+__global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
+  // the particle kernel that follows walks the active tile list from the start
+  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
+    grid_brick(d, d.brick_list[k], lane, step, epoch);
 }
 
 // Scatter one particle's P2G terms of one node: m, m*v, and the internal force
@@ -1045,16 +1074,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // particle reached it, brick flags, epochs ----
     float *pc = d.pcore + (long long)t * CORE_W;
     bool halo = false;
+    static_assert(P2G_NODES % WG == 0, "whole waves per flush round");
     for (int n = tid; n < P2G_NODES; n += WG) {
       const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
       const int a = acc_index(x, y, z);
       const bool shell = x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R;
-      if (!shell) {
+      if (!shell && s_acc[0][a] != 0.0) {  // nodes without mass are never read
         const int c = (x - 1) + CORE_R * ((y - 1) + CORE_R * (z - 1));
-#pragma unroll
-        for (int f = 0; f < 7; f++) pc[f * CORE_NODES + c] = (float)s_acc[f][a];
+        float4 *q = reinterpret_cast<float4 *>(pc + 8 * c);
+        q[0] = make_float4((float)s_acc[0][a], (float)s_acc[1][a], (float)s_acc[2][a], (float)s_acc[3][a]);
+        q[1] = make_float4((float)s_acc[4][a], (float)s_acc[5][a], (float)s_acc[6][a], 0.0f);
       }
-      if (s_acc[0][a] == 0.0) continue;
+      // node mask: the 64 nodes of this wave's round are region nodes n - lane .. n - lane + 63
+      const bool has = s_acc[0][a] != 0.0;
+      const unsigned long long bits = __ballot(has);
+      if ((tid & 63) == 0) d.nodemask[(long long)t * (P2G_NODES / 64) + (n >> 6)] = bits;
+      if (!has) continue;
       halo = halo || shell;
       const int g[3] = {o[0] - 1 + x, o[1] - 1 + y, o[2] - 1 + z};
       if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= I.n[0] || g[1] >= I.n[1] || g[2] >= I.n[2]) continue;
@@ -1068,8 +1103,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
         const int x = n % P2G_R, y = (n / P2G_R) % P2G_R, z = n / (P2G_R * P2G_R);
         if (!(x == 0 || y == 0 || z == 0 || x > CORE_R || y > CORE_R || z > CORE_R)) continue;
         const int a = acc_index(x, y, z);
-#pragma unroll
-        for (int f = 0; f < 7; f++) pt[f * P2G_NODES + n] = (float)s_acc[f][a];
+        if (s_acc[0][a] == 0.0) continue;  // not in the node mask: never read
+        float4 *q = reinterpret_cast<float4 *>(pt + 8 * n);
+        q[0] = make_float4((float)s_acc[0][a], (float)s_acc[1][a], (float)s_acc[2][a], (float)s_acc[3][a]);
+        q[1] = make_float4((float)s_acc[4][a], (float)s_acc[5][a], (float)s_acc[6][a], 0.0f);
       }
       if (tid == 0) d.halo_epoch[t] = epoch_w;
     }
@@ -1549,7 +1586,13 @@ void launch_fused(const DevPtrs &d, hipStream_t st) {
 
 void launch_p2g_only(const DevPtrs &d, void *st) { launch_fused<false>(d, S(st)); }
 void launch_grid(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+  // persistent: at most two workgroups per persistent particle workgroup
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  hipLaunchKernelGGL(k_grid, dim3(nb), dim3(WG), 0, S(st), d);
+}
+void launch_brick_list(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_brick_list_reset, dim3(1), dim3(1), 0, S(st), d);
+  hipLaunchKernelGGL(k_brick_list, dim3(blocks(d.nbricks, WG)), dim3(WG), 0, S(st), d);
 }
 void launch_g2p2g(const DevPtrs &d, void *st) { launch_fused<true>(d, S(st)); }
 void launch_audit(const DevPtrs &d, void *st) {
