@@ -1,0 +1,133 @@
+"""Small seeded scenes behind the golden fixtures in this folder (1,000-2,000 particles,
+a few dozen steps each), one per part of the step the fixtures pin:
+
+  elastic          hypoelastic cube (model 1), gravity, periodic box
+  newtonian        the same cube as a Newtonian liquid (model 2)
+  perfect_plastic  von Mises block (model 3) hitting a grid_vx wall
+  dp_cohesive      Drucker-Prager column (model 4), c = 1 Pa, psi = phi (associative)
+  dp_unguarded     Drucker-Prager column, c = 0, psi = 0, no apex guard: one step, the
+                   reference's NaN apex return (ConstitutiveModels.cpp:347-394)
+  grady_kipp       elastic cube under tension with Grady-Kipp damage (eig3 path)
+  bcs              particle BCs (velocity, force, stress, sine; TimeIntegration.cpp:221-296),
+                   held particles and node BCs, with the time argument set per step
+  rigid_platen     deformable block pressed by a rigid platen (two objects)
+  two_blocks_mu    two deformable halves in contact, sliding, friction coefficient 0.3
+
+`build(name)` returns (scene, steps, t_of); t_of(step) is the time passed to each
+iterate call, or None for step * dt.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from graphyt_amd import workloads as W
+from graphyt_amd.abi import BC, material
+
+NAMES = ["elastic", "newtonian", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
+         "bcs", "rigid_platen", "two_blocks_mu"]
+
+# fields stored per scene (FP64) and compared
+BASE_FIELDS = ["pos", "vel", "stress", "strain", "density", "cell_id"]
+EXTRA_FIELDS = {
+    "perfect_plastic": ["plastic_strain", "acc_plastic_strain"],
+    "dp_cohesive": ["plastic_strain", "acc_plastic_strain"],
+    "dp_unguarded": ["plastic_strain", "acc_plastic_strain"],
+    "grady_kipp": ["damage", "smax"],
+}
+
+
+def fields(name: str) -> list:
+    return BASE_FIELDS + EXTRA_FIELDS.get(name, [])
+
+
+def _cube(steps: int):
+    return W.elastic_cube(side=0.12, steps=steps)  # 12^3 = 1,728 particles
+
+
+def _dp(**kw):
+    return W.dp_column(h=0.02, L=(0.6, 0.5, 0.4), lo=(0.06, 0.06, 0.1), size=(0.1, 0.2, 0.06), **kw)
+
+
+def build(name: str):
+    if name == "elastic":
+        return _cube(80), 80, None
+    if name == "newtonian":
+        sc = _cube(60)
+        sc.mats = [material(model=2, bulk_mod=1e6, viscosity=5.0, density=1000.0)]
+        return sc, 60, None
+    if name == "perfect_plastic":
+        return W.impact_block(n_nodes=16, count=1331, v=150.0), 60, None
+    if name == "dp_cohesive":
+        return _dp(steps=60, cohesion=1.0, psi=30.0, apex_guard=False), 60, None
+    if name == "dp_unguarded":
+        return _dp(steps=1, apex_guard=False), 1, None
+    if name == "grady_kipp":
+        sc = _cube(30)
+        sc.gravity = (0.0, 0.0, 0.0)
+        pos = sc.particles["pos"]
+        vel = np.zeros_like(pos)
+        vel[:, 0] = 20.0 * (pos[:, 0] - 0.25)
+        stress = np.zeros((sc.n, 6))
+        stress[:, 0], stress[:, 1], stress[:, 3] = 3e7, -1e7, 5e6
+        sc.particles.arrays.update(vel=vel, stress=stress)
+        sc.mats = [material(model=1, bulk_mod=3e9, shear_mod=2e9, density=2600.0, is_damage=True,
+                            damage_model=1, crack_speed=500.0, damage_m=9.0, damage_k=1e30)]
+        sc.dt = 1e-7
+        return sc, 30, None
+    if name == "bcs":
+        sc = _cube(40)
+        n = sc.n
+        bct = np.zeros(n, np.int32)
+        bcv = np.zeros((n, 4))
+        codes = [BC["mp_vx"], BC["mp_vy"], BC["mp_vz"], BC["mp_vx_vy"], BC["mp_fx"], BC["mp_fy"],
+                 BC["mp_fz"], BC["mp_fx_fy"], BC["mp_vxsin"], BC["mp_vysin"], BC["mp_syysin"],
+                 BC["mp_stress_xx_yy"]]
+        for i, c in enumerate(codes):
+            bct[i * 37] = c
+            bcv[i * 37] = [0.1 * (i + 1), 2.0 + i, 0.0, 0.0]
+        hold = np.zeros(n, np.uint8)
+        hold[5::301] = 1
+        sc.particles.arrays.update(bc_type=bct, bc_val=bcv, hold=hold)
+        npos = sc.node_positions()
+        t = np.zeros(sc.num_nodes, np.int32)
+        t[npos[:, 2] <= 0.2] = BC["grid_vx_vy"]
+        t[npos[:, 2] >= 0.32] = BC["grid_vy"]
+        t[npos[:, 0] <= 0.2] = BC["grid_vx"]
+        v = np.zeros((sc.num_nodes, 4))
+        v[:, 0], v[:, 1] = 0.05, -0.02
+        sc.node_bc_type, sc.node_bc_val = t, v
+        return sc, 40, (lambda s: 1e-3 * s)
+    if name == "rigid_platen":
+        sc = _cube(80)
+        sc.gravity = (0.0, 0.0, 0.0)
+        pos = sc.particles["pos"]
+        top = pos[:, 2] > np.quantile(pos[:, 2], 0.85)
+        vel = np.zeros_like(pos)
+        vel[top, 2] = -0.3
+        sc.mats = [material(model=1, bulk_mod=1e6, shear_mod=5e5, density=1000.0), material(is_rigid=True)]
+        sc.particles.arrays.update(material=np.where(top, 1, 0).astype(np.int32),
+                                   obj_id=np.where(top, 2, 1).astype(np.int32), vel=vel)
+        return sc, 80, None
+    if name == "two_blocks_mu":
+        sc = _cube(60)
+        sc.gravity = (0.0, 0.0, 0.0)
+        sc.mu = 0.3
+        pos = sc.particles["pos"]
+        left = pos[:, 0] < 0.25
+        vel = np.zeros_like(pos)
+        vel[left, 0], vel[~left, 0] = 0.5, -0.5
+        vel[left, 1], vel[~left, 1] = 0.2, -0.2
+        sc.particles.arrays.update(obj_id=np.where(left, 1, 2).astype(np.int32), vel=vel)
+        return sc, 60, None
+    raise KeyError(name)
+
+
+def input_digest(scene) -> np.ndarray:
+    """A fingerprint of a scene's inputs (particle arrays, node BCs, dt), stored with each
+    fixture so a changed scene builder is caught instead of compared."""
+    parts = [np.asarray(scene.particles.arrays[k], np.float64).ravel() for k in sorted(scene.particles.arrays)]
+    if scene.node_bc_type is not None:
+        parts += [np.asarray(scene.node_bc_type, np.float64), np.asarray(scene.node_bc_val, np.float64).ravel()]
+    v = np.concatenate(parts)
+    return np.array([v.size, np.sum(v), np.sum(v * v), np.sum(v * np.arange(v.size) % 7.0),
+                     scene.dt if scene.dt else 0.0])

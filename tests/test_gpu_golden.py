@@ -1,0 +1,76 @@
+"""The HIP step (through the C ABI) against the golden fixtures written by the reference's
+own compiled code (tests/golden/make_golden.py). Tolerances are relative L2 norms over all
+particles, ||gpu - reference|| / ||reference||, per scene: FP32 state against the FP64
+reference accumulates rounding per step, and plastic and damage branches widen it.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from graphyt_amd import runtime
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import scenes  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+ELASTIC = {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "strain": 1e-4, "density": 1e-6}
+TOL = {
+    "elastic": ELASTIC,
+    "newtonian": ELASTIC,
+    "perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
+    "dp_cohesive": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-3, "acc_plastic_strain": 1e-3},
+    "grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
+    "bcs": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
+    "rigid_platen": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
+    "two_blocks_mu": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
+}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def gpu_run(name):
+    sc, steps, t_of = scenes.build(name)
+    g = runtime.Context(sc)
+    if t_of is None:
+        g.run(steps)
+    else:
+        for s in range(steps):
+            g.iterate(t_of(s), s)
+    out = g.download(scenes.fields(name))
+    g.close()
+    with np.load(os.path.join(HERE, "golden", f"{name}.npz")) as f:
+        fx = {k: f[k] for k in f.files}
+    assert np.allclose(scenes.input_digest(sc), fx["digest"], rtol=1e-12, atol=0), "scene builder changed"
+    return out, fx
+
+
+@pytest.mark.parametrize("name", [n for n in scenes.NAMES if n != "dp_unguarded"])
+def test_gpu_matches_reference_fixture(name):
+    out, fx = gpu_run(name)
+    for k, tol in TOL[name].items():
+        r = rel(out[k], fx[k])
+        assert r <= tol, f"{name} {k}: rel L2 {r:.3e} > {tol:.1e}"
+    # Binning: the cube lattices sit exactly on cell faces (x - 0.25h a multiple of h), so a
+    # cell id is only rounding-proof along axes the particles do not move. In the elastic
+    # and Newtonian cubes only z moves and every id must agree; elsewhere ids of particles
+    # on a face legitimately follow FP32 vs FP64 rounding.
+    if name in ("elastic", "newtonian"):
+        assert np.array_equal(out["cell_id"], fx["cell_id"])
+
+
+def test_gpu_unguarded_apex_nan_share():
+    """c = 0, psi = 0, no guard: the reference's apex return is NaN (0*inf) for yielding
+    particles in tension. The mean stress at step 0 is rounding noise, so which particles
+    go NaN depends on FP64 vs FP32 rounding; the share must be comparable, and the finite
+    particles agree."""
+    out, fx = gpu_run("dp_unguarded")
+    na, nb = np.isnan(out["stress"]).any(axis=1), np.isnan(fx["stress"]).any(axis=1)
+    assert nb.sum() > 0 and 0.5 * nb.sum() < na.sum() < 2.0 * nb.sum()
+    both = ~na & ~nb
+    assert rel(out["pos"][both], fx["pos"][both]) <= 1e-6
