@@ -114,12 +114,10 @@ struct Ctrl {
   int oor;           // count of out-of-range particles (clamped)
   int epoch;         // tile partials written with tile_epoch == epoch are this step's P2G
   int active_tiles;  // tiles holding particles in the current binning
-  int xlo[9];        // per-XCD ranges [xlo[g], xlo[g+1]) of the active tiles in tile order
-  int hxlo[9];       // heavy tiles (more than one chunk) before xlo[g], in tile order
-  // per-XCD work counters of the persistent particle kernels, on a line of their own:
-  // every workgroup adds to them, and loads of fields sharing their line stall behind
-  // those atomics
-  alignas(256) int next[8];
+  int heavy_tiles;   // active tiles holding more than one chunk of entries
+  // work counter of the persistent particle kernels, on a line of its own: every
+  // workgroup adds to it, and loads of fields sharing its line stall behind those atomics
+  alignas(256) int next;
   int active_bricks;  // length of the flagged-brick list (appended with atomics)
 };
 

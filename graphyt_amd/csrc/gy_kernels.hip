@@ -1,18 +1,21 @@
 // gy_kernels.hip — HIP kernels (gfx950) of the MPM time step.
 //
 // One step is five launches on one stream:
-//   k_grid      nodal solve on the active canonical bricks: gather the P2G tile partials
-//               of every node of the periodic class, v = p/m, a = f/m + g, body-1 kick
-//               (half at step 1), BCs, fan-out to every mirror node
-//               (Grid.cpp:9-245, 310-603, 862-1094)
-//   k_g2p2g     persistent over the active cell tiles: G2P from LDS-staged node
+//   k_grid      nodal solve, persistent over the list of flagged canonical bricks: gather
+//               the P2G tile partials of every node of the periodic class, v = p/m,
+//               a = f/m + g, body-1 kick (half at step 1), BCs, fan-out to every mirror
+//               node (Grid.cpp:9-245, 310-603, 862-1094)
+//   k_g2p2g     persistent over the active cell tiles (one work queue, tiles with more
+//               than one chunk of particles first): G2P from LDS-staged node
 //               velocities, particle update, constitutive/damage models, periodic wrap,
 //               new cell key + histogram, and P2G of the updated particle into an FP64 LDS
 //               region with a one-node halo, written out as per-tile FP32 partial sums
 //               with plain stores (Extrapolation.cpp:31-367, TimeIntegration.cpp:11-296,
 //               ConstitutiveModels.cpp, DamageModels.cpp)
-//   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets); the
-//               first also does the step/parity/time bookkeeping
+//   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets), the
+//               active tile list (heavy tiles from the front, light from the back) and
+//               the flagged-brick list; the first also does the step/parity/time
+//               bookkeeping
 //   k_place     cell-grouped slot list (within a cell in histogram order; the stable
 //               order, ascending slot, is computed only for gy_debug_binning and
 //               gy_counting_sort)
@@ -38,14 +41,6 @@ constexpr int NODES_P2G = P2G_NODES;
 
 __device__ __forceinline__ float f32(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
-
-// Contiguous tile ranges per XCD: blocks are dispatched round-robin over the 8 XCDs, so
-// block b goes to XCD b % 8; map it to a tile inside that XCD's contiguous chunk, which
-// keeps neighbouring tiles (shared halo nodes) in one L2.
-__device__ __forceinline__ int xcd_remap(int b, int G) {
-  const int q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
-}
 
 // Stencil base cell of a fixed-point coordinate, clamped to the grid (getGIMPNodes,
 // Extrapolation.cpp:658-660: int((x - psep/2 - x0)/h)); `bad` is set when clamped.
@@ -255,28 +250,22 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
   }
 }
 
-// A tile's fixed cost in the particle kernel (set-up, flush), in particle equivalents:
-// the weight by which the active tile list is cut into equal per-XCD shares.
-constexpr int TILE_COST = 192;
-
 __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   // prefixes of all earlier blocks (particles and active tiles), and the totals
-  int pre = 0, pret = 0, preh = 0, all = 0, allt = 0, allh = 0;
+  int pre = 0, pret = 0, preh = 0, allt = 0, allh = 0;
   for (int j = threadIdx.x; j < (int)gridDim.x; j += WG) {
-    const int bs = d.blocksum[j], bt = d.blocktiles[j] & 0xffff, bh = d.blocktiles[j] >> 16;
+    const int bt = d.blocktiles[j] & 0xffff, bh = d.blocktiles[j] >> 16;
     if (j < (int)blockIdx.x) {
-      pre += bs;
+      pre += d.blocksum[j];
       pret += bt;
       preh += bh;
     }
-    all += bs;
     allt += bt;
     allh += bh;
   }
   pre = block_sum(pre);
   pret = block_sum(pret);
   preh = block_sum(preh);
-  all = block_sum(all);
   allt = block_sum(allt);
   allh = block_sum(allh);
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
@@ -291,7 +280,6 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   }
   int tot;
   int run = pre + block_exclusive_scan(s, tot);
-  const int run0 = run;  // first entry of this thread's keys
   if (have) {
     int4 *ov = reinterpret_cast<int4 *>(d.offsets + base);
     int4 *cv = reinterpret_cast<int4 *>(d.count + base);
@@ -316,32 +304,12 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   const int pos = pret + block_exclusive_scan(act, ttot);  // index in tile order
   const int hpos = preh + block_exclusive_scan(heavy, htot);  // heavy tiles before it
   if (act) d.active[heavy ? hpos : allt - 1 - (pos - hpos)] = (int)(base / TILE3);
-  // per-XCD ranges of the active list for the particle kernel: equal shares of the
-  // weight (particles + TILE_COST per tile), cut at tile boundaries. Active tile `pos`
-  // spans weights [w0, w1); the tile holding g/8 of the total starts range g.
-  if (act) {
-    const long long wt = (long long)all + (long long)TILE_COST * allt;
-    const long long w0 = (long long)run0 + (long long)TILE_COST * pos, w1 = w0 + ts + TILE_COST;
-    for (int g = 1; g < 8; g++) {
-      const long long target = g * wt / 8;
-      if (w0 <= target && target < w1) {
-        d.ctrl->xlo[g] = pos;
-        d.ctrl->hxlo[g] = hpos;
-      }
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d.ctrl->xlo[0] = d.ctrl->hxlo[0] = 0;
-    d.ctrl->xlo[8] = allt;
-    d.ctrl->hxlo[8] = allh;
-    if (allt == 0)
-      for (int g = 1; g < 8; g++) d.ctrl->xlo[g] = d.ctrl->hxlo[g] = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allh;
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     d.offsets[d.nkeys] = pre + tot;
     d.ctrl->active_tiles = pret + ttot;
   }
-  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   // list of the bricks the particle kernel flagged, for the next grid update
   append_bricks(d, blockIdx.x * WG + threadIdx.x, gridDim.x * WG);
 }
@@ -501,7 +469,7 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
 // Grid update, persistent over the list of flagged bricks.
 __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
   // the particle kernel that follows walks the active tile list from the start
-  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
@@ -946,22 +914,17 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 // chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
 // is stamped; the grid kernel gathers the partials of the tiles covering each node in a
 // fixed order.
-// The active tiles of XCD group g: its heavy tiles first (in tile order), then its light
-// ones, so the claims that end the launch are the short ones. Read once per launch.
+// The particle kernels' work queue over the active tiles: heavy tiles first (in tile
+// order), then the light ones, so the claims that end a launch are the short ones; one
+// counter for the whole device (balance across the XCDs matters more here than keeping
+// neighbouring tiles in one L2: measured). Bounds read once per launch.
 struct TileQueue {
-  int h0, nh, l0, n, nact;
-  __device__ __forceinline__ TileQueue(const DevPtrs &d, int g) {
-    const int x0 = d.ctrl->xlo[g], x1 = d.ctrl->xlo[g + 1];
-    h0 = d.ctrl->hxlo[g];
-    nh = d.ctrl->hxlo[g + 1] - h0;
-    l0 = x0 - h0;  // light tiles before the range
-    n = x1 - x0;
-    nact = d.ctrl->active_tiles;
-  }
+  int nh, n;
+  __device__ __forceinline__ explicit TileQueue(const DevPtrs &d) : nh(d.ctrl->heavy_tiles), n(d.ctrl->active_tiles) {}
   // tile of claim k, -1 past the end
   __device__ __forceinline__ int tile(const DevPtrs &d, int k) const {
-    if (k < nh) return d.active[h0 + k];
-    if (k < n) return d.active[nact - 1 - (l0 + k - nh)];
+    if (k < nh) return d.active[k];
+    if (k < n) return d.active[n - 1 - (k - nh)];
     return -1;
   }
 };
@@ -976,14 +939,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ MatDev s_mat[MAXMAT];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
-  const int grp = blockIdx.x & 7;  // dispatch XCD of this workgroup (round-robin)
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
-  const TileQueue queue(d, grp);
+  const TileQueue queue(d);
 
   for (;;) {
     if (tid == 0) {
-      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next[grp], 1));
+      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
     }
     __syncthreads();
     // workgroup-uniform values read from LDS: keep them (and all that derives from
@@ -1277,7 +1239,7 @@ __device__ __forceinline__ void contact_normal(const float rel[3], const float n
 // (Grid.cpp:967-1031). VEL gets the centre-of-mass velocity p/m, its acceleration and m;
 // VELB the body velocities and accelerations G2P reads.
 __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
-  if (blockIdx.x == 0 && threadIdx.x < 8) d.ctrl->next[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gb = blockIdx.x * (WG / 64) + wave;
   if (gb >= d.nbricks) return;
@@ -1438,12 +1400,11 @@ __global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
   __shared__ int s_off[TILE3 + 1];
   __shared__ int s_tile;
   const int tid = threadIdx.x;
-  const int grp = blockIdx.x & 7;
   const int par = d.ctrl->parity;
-  const TileQueue queue(d, grp);
+  const TileQueue queue(d);
   for (;;) {
     if (tid == 0) {
-      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next[grp], 1));
+      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
     }
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_tile);
