@@ -96,13 +96,21 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
     return 0
 
 
+ENSEMBLE_STEPS = 500  # the batched-impact configuration is quoted over 500 steps
+
+
 def bench_ensemble(args) -> int:
     """256 independent perfect-plastic block impacts advanced together (one context, one
-    graph per step); value = total particle-steps/s over all instances."""
+    graph per step); value = total particle-steps/s over all instances. The warm-up and the
+    two timed runs (events, then graph replay) stay within the configuration's 500 steps:
+    later the fastest blocks, rebounding from the wall, leave their small grids."""
     from graphyt_amd import runtime, workloads as W
     scenes = W.impact_ensemble()
     n = sum(sc.n for sc in scenes)
     log(f"workload impact_ensemble: {len(scenes)} instances, {n} particles")
+    if args.warmup + 2 * args.steps > ENSEMBLE_STEPS:
+        args.steps = max(1, (ENSEMBLE_STEPS - args.warmup) // 2)
+        log(f"timed steps per run capped at {args.steps} ({ENSEMBLE_STEPS}-step configuration)")
     ctx = runtime.BatchContext(scenes)
     tuned = ctx.autotune(10) if args.autotune else None
     if tuned:
