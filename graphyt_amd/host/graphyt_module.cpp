@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -443,6 +444,134 @@ class Solver {
   }
   bool has_context() const { return ctx != nullptr; }
 
+  // Binary checkpoint of the state of the last step (the format of
+  // graphyt_amd/checkpoint.py): particle arrays, node BCs, grid, t, step, dt.
+  void checkpoint(const std::string &path, double t, int64_t step) {
+    Synchronise();
+    struct Arr { const char *name; int code; size_t count; const void *data; };
+    const auto &P = *mpts;
+    std::vector<Arr> arrs = {
+        {"pos", 0, P.position.size(), P.position.data()}, {"pos0", 0, P.pos0.size(), P.pos0.data()},
+        {"vel", 0, P.velocity.size(), P.velocity.data()}, {"stress", 0, P.stress.size(), P.stress.data()},
+        {"strain", 0, P.strain.size(), P.strain.data()},
+        {"plastic_strain", 0, P.plastic_strain.size(), P.plastic_strain.data()},
+        {"acc_plastic_strain", 0, P.acc_plastic_strain.size(), P.acc_plastic_strain.data()},
+        {"density", 0, P.density.size(), P.density.data()}, {"mass", 0, P.mass.size(), P.mass.data()},
+        {"volume", 0, P.volume.size(), P.volume.data()}, {"damage", 0, P.damage.size(), P.damage.data()},
+        {"smax", 0, P.smax.size(), P.smax.data()}, {"material", 1, P.material.size(), P.material.data()},
+        {"obj_id", 1, P.objID.size(), P.objID.data()}, {"hold", 2, P.hold.size(), P.hold.data()},
+        {"bc_type", 1, P.BCTYPE.size(), P.BCTYPE.data()}, {"bc_val", 0, P.BCVAL.size(), P.BCVAL.data()},
+        {"cell_id", 1, P.cellID.size(), P.cellID.data()},
+        {"node_bc_type", 1, nds->BCTYPE.size(), nds->BCTYPE.data()},
+        {"node_bc_val", 0, nds->BCVAL.size(), nds->BCVAL.data()},
+        {"grid_L", 0, nds->L.size(), nds->L.data()}, {"grid_h", 0, 1, &nds->cellsize},
+        {"psep", 0, 1, &P.psep}};
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot open " + path);
+    const size_t esize[3] = {8, 4, 1};
+    bool ok = std::fwrite("GYCKPT\x00\x01", 1, 8, fp) == 8;
+    const uint32_t head[2] = {1u, (uint32_t)arrs.size()};
+    const double td[2] = {t, dt};
+    ok = ok && std::fwrite(head, 4, 2, fp) == 2 && std::fwrite(td, 8, 2, fp) == 2 && std::fwrite(&step, 8, 1, fp) == 1;
+    for (const auto &a : arrs) {
+      const uint32_t ln = (uint32_t)std::strlen(a.name);
+      const uint8_t code = (uint8_t)a.code;
+      const uint64_t count = a.count;
+      ok = ok && std::fwrite(&ln, 4, 1, fp) == 1 && std::fwrite(a.name, 1, ln, fp) == ln &&
+           std::fwrite(&code, 1, 1, fp) == 1 && std::fwrite(&count, 8, 1, fp) == 1 &&
+           (count == 0 || std::fwrite(a.data, esize[a.code], count, fp) == count);
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) throw std::runtime_error("writing " + path + " failed");
+  }
+
+  // Restore a checkpoint written for the same setup (grid, particle count, materials):
+  // the host arrays are replaced and uploaded before the next step. Returns (t, step).
+  std::pair<double, int64_t> restore(const std::string &path) {
+    FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) throw std::runtime_error("cannot open " + path);
+    std::vector<char> data;
+    {
+      char buf[1 << 16];
+      size_t k;
+      while ((k = std::fread(buf, 1, sizeof buf, fp)) > 0) data.insert(data.end(), buf, buf + k);
+      std::fclose(fp);
+    }
+    size_t off = 0;
+    auto take = [&](void *dst, size_t n) {
+      if (off + n > data.size()) throw std::runtime_error(path + ": truncated checkpoint");
+      std::memcpy(dst, data.data() + off, n);
+      off += n;
+    };
+    char magic[8];
+    take(magic, 8);
+    if (std::memcmp(magic, "GYCKPT\x00\x01", 8) != 0) throw std::runtime_error(path + ": not a graphyt checkpoint");
+    uint32_t head[2];
+    take(head, 8);
+    if (head[0] != 1u) throw std::runtime_error(path + ": unsupported checkpoint version");
+    double td[2];
+    int64_t step = 0;
+    take(td, 16);
+    take(&step, 8);
+    Synchronise();  // nothing the device holds is newer than what the checkpoint replaces
+    auto &P = *mpts;
+    const size_t n = P.size();
+    const size_t esize[3] = {8, 4, 1};
+    for (uint32_t a = 0; a < head[1]; a++) {
+      uint32_t ln;
+      take(&ln, 4);
+      if (ln > 256) throw std::runtime_error(path + ": bad array name");
+      std::string name(ln, '\0');
+      take(&name[0], ln);
+      uint8_t code;
+      uint64_t count;
+      take(&code, 1);
+      take(&count, 8);
+      if (code > 2) throw std::runtime_error(path + ": bad array type");
+      const size_t nbytes = (size_t)count * esize[code];
+      if (off + nbytes > data.size()) throw std::runtime_error(path + ": truncated array " + name);
+      const char *src = data.data() + off;
+      off += nbytes;
+      auto fill = [&](auto &vec, int want_code, size_t want) {
+        if (code != want_code || count != want) throw std::runtime_error(path + ": " + name + " does not fit this setup");
+        std::memcpy(vec.data(), src, nbytes);
+      };
+      if (name == "pos") fill(P.position, 0, 3 * n);
+      else if (name == "pos0") fill(P.pos0, 0, 3 * n);
+      else if (name == "vel") fill(P.velocity, 0, 3 * n);
+      else if (name == "stress") fill(P.stress, 0, 6 * n);
+      else if (name == "strain") fill(P.strain, 0, 6 * n);
+      else if (name == "plastic_strain") fill(P.plastic_strain, 0, 6 * n);
+      else if (name == "acc_plastic_strain") fill(P.acc_plastic_strain, 0, n);
+      else if (name == "density") fill(P.density, 0, n);
+      else if (name == "mass") fill(P.mass, 0, n);
+      else if (name == "volume") fill(P.volume, 0, n);
+      else if (name == "damage") fill(P.damage, 0, n);
+      else if (name == "smax") fill(P.smax, 0, n);
+      else if (name == "material") fill(P.material, 1, n);
+      else if (name == "obj_id") fill(P.objID, 1, n);
+      else if (name == "hold") fill(P.hold, 2, n);
+      else if (name == "bc_type") fill(P.BCTYPE, 1, n);
+      else if (name == "bc_val") fill(P.BCVAL, 0, 4 * n);
+      else if (name == "cell_id") fill(P.cellID, 1, n);
+      else if (name == "node_bc_type") fill(nds->BCTYPE, 1, (size_t)nds->numNodes);
+      else if (name == "node_bc_val") fill(nds->BCVAL, 0, 4 * (size_t)nds->numNodes);
+      else if (name == "grid_h") {
+        double h;
+        std::memcpy(&h, src, 8);
+        if (code != 0 || count != 1 || h != nds->cellsize) throw std::runtime_error(path + ": grid spacing differs");
+      }
+    }
+    for (size_t p = 0; p < n; p++)
+      for (int k = 0; k < 3; k++) P.displacement[3 * p + k] = P.position[3 * p + k] - P.pos0[3 * p + k];
+    dt = td[1];
+    if (ctx) check(ctx, gy_set_dt(ctx, dt), "gy_set_dt");
+    P.host_dirty = true;
+    P.device_newer = false;
+    nds->bc_dirty = true;
+    return {td[0], step};
+  }
+
   Nodes *nds;
   MaterialPoints *mpts;
   Parameters *prm;
@@ -582,6 +711,7 @@ PYBIND11_MODULE(graphyt, m) {
 
   py::class_<Nodes>(m, "Nodes")
       .def(py::init<std::vector<double>, double>(), py::arg("L"), py::arg("cellsize"))
+      .def("_sync", &Nodes::pull)  // node fields of the last step into the host arrays (writers call it)
       .def("setBC", &Nodes::setBC, py::arg("nIdx"), py::arg("type"), py::arg("val"))
       .def("Serialise", &Nodes::Serialise, py::arg("step"), py::arg("filename"))
       .def("getPosX", &Nodes::getPosX)
@@ -614,6 +744,7 @@ PYBIND11_MODULE(graphyt, m) {
   };
   py::class_<MaterialPoints>(m, "MaterialPoints")
       .def(py::init<Nodes *>(), py::keep_alive<1, 2>())
+      .def("_sync", &MaterialPoints::pull)  // particle state of the last step into the host arrays
       .def("add", &MaterialPoints::add, py::arg("i"), py::arg("x"), py::arg("y"), py::arg("z"))
       .def("addPyckGeom", &MaterialPoints::addPyckGeom, py::arg("pos"), py::arg("numParticle"))
       .def("getPosX", &MaterialPoints::getPosX)
@@ -674,6 +805,8 @@ PYBIND11_MODULE(graphyt, m) {
       .def("iterate_GPU_steps", &Solver::iterate_GPU_steps, py::arg("nsteps"), py::arg("parameters"))
       .def("setDt", &Solver::setDt)
       .def("Synchronise", &Solver::Synchronise)
+      .def("checkpoint", &Solver::checkpoint, py::arg("path"), py::arg("t"), py::arg("step"))
+      .def("restore", &Solver::restore, py::arg("path"))
       .def("initialize", &Solver::initialize)
       .def_readonly("dt", &Solver::dt)
       .def_readwrite("tmax", &Solver::tmax)
