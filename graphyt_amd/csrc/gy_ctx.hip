@@ -151,7 +151,8 @@ MatDev make_mat(const gy_material &m) {
   M.gk = (m.is_damage != 0) && m.damage_model == 1;
   M.apex_guard = m.dp_apex_guard != 0;
   M.carry_plastic = (m.model == 3 || m.model == 4);
-  M.carry_damage = M.gk;
+  M.carry_damage = M.gk || m.model == 5;  // model 5 keeps its turbulence flag in the damage field
+  M.turb = (float)m.turb_stress;
   M.K = (float)m.bulk_mod;
   M.G = (float)m.shear_mod;
   M.visc = (float)m.viscosity;
@@ -202,7 +203,7 @@ int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, con
   if (!(p->psep > 0) || p->psep > h) return fail(c, GY_ERR_UNSUPPORTED, "psep must be in (0, cellsize]");
   for (int i = 0; i < nmat; i++) {
     const int md = mats[i].model;
-    if (md < 1 || md > 4) return fail(c, GY_ERR_UNSUPPORTED, "material %d: model %d not implemented", i, md);
+    if (md < 1 || md > 5) return fail(c, GY_ERR_UNSUPPORTED, "material %d: model %d not implemented", i, md);
   }
   H.grid = *g;
   H.params = *p;
@@ -565,7 +566,7 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     for (int m = 0; m < nmat; m++) {
       if (!used[m]) continue;
       const gy_material &gm = H.mats[m];
-      kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : 8;
+      kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : gm.model == 4 ? 8 : 64;
       if (gm.is_damage && gm.damage_model == 1) kmask |= 16;
       if (c->mats[I.mat_base + m].carry_damage && !(gm.is_damage && gm.damage_model == 1)) kmask |= 16;
     }

@@ -89,14 +89,15 @@ struct InstDesc {
 // Per-material constants, derived on the host in FP64 (ConstitutiveModels.cpp,
 // DamageModels.cpp) and stored in FP32 except for the Grady-Kipp block.
 struct MatDev {
-  int model;        // 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager
+  int model;        // 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager, 5 turbulent liquid
   int rigid;
   int damage;       // isDamage (also switches the Drucker-Prager criterion)
   int gk;           // Grady-Kipp damage update runs (isDamage && damageModel == 1)
   int apex_guard;
   int carry_plastic;  // plastic strain fields are live for this material
-  int carry_damage;   // damage / smax fields are live for this material
+  int carry_damage;   // damage / smax fields are live for this material (Grady-Kipp, model 5's flag)
   float K, G, visc, sy;
+  float turb;         // turbStress of model 5
   float c, ecr, eta, xi, eta_bar;
   float c_over_ecr, c_floor;   // c/eps_cr, c/10
   float xi_over_eta, xi_over_etab;

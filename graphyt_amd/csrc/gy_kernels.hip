@@ -533,8 +533,9 @@ __device__ __noinline__ void p2g_global(float *acc, uint8_t *brick_flag, const I
 
 // Model-set bits: the fused kernel is instantiated per set of constitutive models a
 // context uses, so a single-material run carries only its own model's registers.
-constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32;
-constexpr int KM_ALL = 63;
+constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32,
+              KM_TURB = 64;
+constexpr int KM_ALL = 127;
 
 // The fused kernel works through a tile's entries in chunks of CHUNK list entries.
 constexpr int CHUNK = CHUNK_ENTRIES;
@@ -757,10 +758,11 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = u32(p[a]);
         B[F_EPB * cap + out] = u32(epb);
       }
-      if (MASK & KM_GK) {
+      if (MASK & (KM_GK | KM_TURB)) {
         if (M.carry_damage) {
           float dmg = f32(A[F_DMG * cap + slot]), smax = f32(A[F_SMAX * cap + slot]);
-          if (M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
+          if ((MASK & KM_TURB) && M.model == 5) cm_turbulent(M, er, e, v, mass, dmg, s, dt);
+          if ((MASK & KM_GK) && M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
           B[F_DMG * cap + out] = u32(dmg);
           B[F_SMAX * cap + out] = u32(smax);
         }
@@ -783,7 +785,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = A[(F_P0 + a) * cap + slot];
       B[F_EPB * cap + out] = A[F_EPB * cap + slot];
     }
-    if (!(MASK & KM_GK) && M.carry_damage && !M.rigid) {
+    if (!(MASK & (KM_GK | KM_TURB)) && M.carry_damage && !M.rigid) {
       B[F_DMG * cap + out] = A[F_DMG * cap + slot];
       B[F_SMAX * cap + out] = A[F_SMAX * cap + slot];
     }

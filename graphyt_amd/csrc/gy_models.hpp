@@ -41,6 +41,33 @@ __device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6],
   s[5] = 2.0f * M.visc * er[5] * dt;
 }
 
+// turbulentLiquid, ConstitutiveModels.cpp:66-126. As there, the deviatoric rate is formed
+// while the volumetric part is still zero, so it is the full rate; the damage field holds
+// the turbulence flag: a set flag halves the viscosity of this update, then the flag is
+// set when KE - VE - turbStress > 0 and cleared (with the viscosity restored) otherwise.
+// KE = m |v|^2 / 2 with the velocity after this step's update, VE = mu (shear rates^2) / 2.
+__device__ __forceinline__ void cm_turbulent(const MatDev &M, const float er[6], const float e[6],
+                                             const float v[3], float mass, float &dmg, float s[6], float dt) {
+  float mu = M.visc;
+  if (dmg == 1.0f) mu = M.visc / 2.0f;
+  const float KE = 0.5f * mass * ((v[0] * v[0]) + (v[1] * v[1]) + (v[2] * v[2]));
+  const float VE = 0.5f * M.visc * (er[3] * er[3] + er[4] * er[4] + er[5] * er[5]);
+  const float crit = KE - VE - M.turb;
+  if (crit > 0.0f) dmg = 1.0f;
+  if (crit <= 0.0f) {
+    dmg = 0.0f;
+    mu = M.visc;
+  }
+  const float P = M.K * (e[0] + e[1] + e[2]);
+  const float c = mu * (2.0f - (2.0f / 3.0f));
+  s[0] = P + c * er[0] * dt;
+  s[1] = P + c * er[1] * dt;
+  s[2] = P + c * er[2] * dt;
+  s[3] = 2.0f * mu * er[3] * dt;
+  s[4] = 2.0f * mu * er[4] * dt;
+  s[5] = 2.0f * mu * er[5] * dt;
+}
+
 // perfectPlasticity, ConstitutiveModels.cpp:131-234
 __device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], float s[6], float p[6],
                                            float &epb, float dt) {

@@ -10,10 +10,10 @@
 // tests/test_oracle_vs_reference.py against oracle/_ref). Every function cites the
 // reference lines it follows (paths relative to the Graphyt repository root).
 //
-// Scope: 3D only; constitutive models 1-4; Grady-Kipp damage (3D branch); one- and
+// Scope: 3D only; constitutive models 1-5; Grady-Kipp damage (3D branch); one- and
 // two-body contact with rigid bodies; grid and particle BCs; periodic wrap. Thermal
-// fields are not simulated (T stays 0, as in every configuration studied here); the
-// turbulent-liquid model (5) and the 2D branches are out of scope.
+// fields are not simulated (T stays 0, as in every configuration studied here); the 2D
+// branches are out of scope.
 //
 // Parity notes:
 //  * Grady-Kipp uses a cyclic Jacobi eigensolver (jacobi_eig3 below) instead of the
@@ -494,6 +494,34 @@ void newtonian(const gy_material &M, const double *er, const double *eps, double
   st[5] = 2.0 * mu * (dev[5]) * dt;
 }
 
+// turbulentLiquid, ConstitutiveModels.cpp:66-126 (3D). As there, the deviatoric rate is
+// formed while the volumetric part is still zero, so it is the full rate; the damage
+// field holds the turbulence flag: a set flag halves the viscosity of this update, then
+// the flag is set when KE - VE - turbStress > 0 and cleared (with the viscosity restored)
+// otherwise. KE uses the velocity and mass after this step's velocity update.
+void turbulent(const gy_material &M, const double *er, const double *eps, const double *vel, double mass,
+               double &dmg, double *st, double dt) {
+  const double K = M.bulk_mod, mu_mat = M.viscosity, turb = M.turb_stress;
+  double mu = mu_mat;
+  const double dev[6] = {er[0] - 0.0, er[1] - 0.0, er[2] - 0.0, er[3], er[4], er[5]};
+  if (dmg == 1) mu = mu_mat / 2.0;
+  const double KE = 0.5 * mass * ((vel[0] * vel[0]) + (vel[1] * vel[1]) + (vel[2] * vel[2]));
+  const double VE = 0.5 * mu_mat * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5]);
+  const double crit = KE - VE - turb;
+  if (crit > 0) dmg = 1.0;
+  if (crit <= 0) {
+    dmg = 0.0;
+    mu = mu_mat;
+  }
+  const double P = K * (eps[0] + eps[1] + eps[2]);
+  st[0] = P + mu * (2. - (2. / 3.)) * (dev[0]) * dt;
+  st[1] = P + mu * (2. - (2. / 3.)) * (dev[1]) * dt;
+  st[3] = 2.0 * mu * (dev[3]) * dt;
+  st[2] = P + mu * (2. - (2. / 3.)) * (dev[2]) * dt;
+  st[4] = 2.0 * mu * (dev[4]) * dt;
+  st[5] = 2.0 * mu * (dev[5]) * dt;
+}
+
 // perfectPlasticity, ConstitutiveModels.cpp:131-234 (3D)
 void perfect_plastic(const gy_material &M, const double *er, double *st, double *ep, double &epbar,
                      double dt) {
@@ -749,6 +777,7 @@ void update_particle(Sim &s, int mp) {
       case 2: newtonian(M, er, &s.eps[6 * mp], st, dt); break;
       case 3: perfect_plastic(M, er, st, &s.epsp[6 * mp], s.epbar[mp], dt); break;
       case 4: drucker_prager(M, er, st, &s.epsp[6 * mp], s.epbar[mp], dt); break;
+      case 5: turbulent(M, er, &s.eps[6 * mp], &s.vel[3 * mp], s.mass[mp], s.dmg[mp], st, dt); break;
       default: break;
     }
     if (M.is_damage && M.damage_model == 1)

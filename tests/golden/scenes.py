@@ -3,6 +3,8 @@ a few dozen steps each), one per part of the step the fixtures pin:
 
   elastic          hypoelastic cube (model 1), gravity, periodic box
   newtonian        the same cube as a Newtonian liquid (model 2)
+  turbulent        the same cube as a turbulent liquid (model 5): the turbulence flag in
+                   the damage field switches where KE > VE + turbStress
   perfect_plastic  von Mises block (model 3) hitting a grid_vx wall
   dp_cohesive      Drucker-Prager column (model 4), c = 1 Pa, psi = phi (associative)
   dp_unguarded     Drucker-Prager column, c = 0, psi = 0, no apex guard: one step, the
@@ -23,7 +25,7 @@ import numpy as np
 from graphyt_amd import workloads as W
 from graphyt_amd.abi import BC, material
 
-NAMES = ["elastic", "newtonian", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
+NAMES = ["elastic", "newtonian", "turbulent", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
          "bcs", "rigid_platen", "two_blocks_mu"]
 
 # fields stored per scene (FP64) and compared
@@ -33,6 +35,7 @@ EXTRA_FIELDS = {
     "dp_cohesive": ["plastic_strain", "acc_plastic_strain"],
     "dp_unguarded": ["plastic_strain", "acc_plastic_strain"],
     "grady_kipp": ["damage", "smax"],
+    "turbulent": ["damage"],
 }
 
 
@@ -54,6 +57,10 @@ def build(name: str):
     if name == "newtonian":
         sc = _cube(60)
         sc.mats = [material(model=2, bulk_mod=1e6, viscosity=5.0, density=1000.0)]
+        return sc, 60, None
+    if name == "turbulent":
+        sc = _cube(60)
+        sc.mats = [material(model=5, bulk_mod=1e6, viscosity=1e-3, density=1000.0, turb_stress=2e-5)]
         return sc, 60, None
     if name == "perfect_plastic":
         return W.impact_block(n_nodes=16, count=1331, v=150.0), 60, None

@@ -52,6 +52,8 @@ def test_fixtures_exercise_their_branch():
     assert (load("dp_cohesive")["acc_plastic_strain"] > 0).sum() > 10
     nan = np.isnan(load("dp_unguarded")["stress"]).any(axis=1)
     assert 0 < nan.sum() < nan.size  # the 0*inf apex return of the reference
+    flag = load("turbulent")["damage"]
+    assert 0 < (flag == 1.0).sum() < flag.size and set(np.unique(flag)) <= {0.0, 1.0}
     gk = load("grady_kipp")
     assert (gk["damage"] > 0).sum() > 10 and (gk["smax"] > 0).all()
     sc, steps, _ = scenes.build("rigid_platen")

@@ -21,6 +21,7 @@ ELASTIC = {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "strain": 1e-4, "density": 
 TOL = {
     "elastic": ELASTIC,
     "newtonian": ELASTIC,
+    "turbulent": ELASTIC,
     "perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
     "dp_cohesive": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-3, "acc_plastic_strain": 1e-3},
     "grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
@@ -56,6 +57,10 @@ def test_gpu_matches_reference_fixture(name):
     for k, tol in TOL[name].items():
         r = rel(out[k], fx[k])
         assert r <= tol, f"{name} {k}: rel L2 {r:.3e} > {tol:.1e}"
+    if name == "turbulent":
+        # the flag follows KE - VE - turbStress > 0, evaluated in FP32 here and FP64 there:
+        # only particles within rounding of the threshold may differ
+        assert np.sum(out["damage"] != fx["damage"]) <= max(2, out["damage"].size // 100)
     # Binning: the cube lattices sit exactly on cell faces (x - 0.25h a multiple of h), so a
     # cell id is only rounding-proof along axes the particles do not move. In the elastic
     # and Newtonian cubes only z moves and every id must agree; elsewhere ids of particles

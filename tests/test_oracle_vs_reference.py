@@ -75,6 +75,17 @@ def test_newtonian_bit_identical(ref):
     _assert_identical(ra, rb)
 
 
+def test_turbulent_liquid_bit_identical(ref):
+    """Model 5: the turbulence flag (damage field) switches on where the kinetic energy
+    exceeds the viscous energy plus turbStress, and off again where it falls below."""
+    sc = W.elastic_cube(steps=40)
+    sc.mats = [material(model=5, bulk_mod=1e6, viscosity=1e-3, density=1000.0, turb_stress=2e-5)]
+    ra, rb, a, b = _pair(ref, sc, 40)
+    _assert_identical(ra, rb)
+    flag = ra["damage"]
+    assert 0 < (flag == 1.0).sum() < flag.size and set(np.unique(flag)) <= {0.0, 1.0}
+
+
 def test_particle_bcs_and_hold_bit_identical(ref):
     sc = W.elastic_cube(steps=20)
     n = sc.n
