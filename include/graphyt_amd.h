@@ -60,7 +60,8 @@ typedef struct gy_grid {
 
 /* One MaterialModels entry (MaterialModels.hpp:5-48, defaults MaterialModels.cpp:7-25). */
 typedef struct gy_material {
-  int32_t model;          /* 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager */
+  int32_t model;          /* 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager,
+                             5 turbulent liquid (its flag lives in the damage field) */
   int32_t is_rigid;
   int32_t is_damage;
   int32_t damage_model;   /* 1 = Grady-Kipp */
