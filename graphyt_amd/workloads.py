@@ -10,6 +10,9 @@ Random draws use numpy's default_rng with a recorded seed.
                           (2x1x1 m, h = 0.01, 80x160x80 = 1,024,000 particles, 1000 steps),
                           and scaled-down versions of it for CPU parity runs.
   * `impact_block`      — elastic/perfect-plastic block impacting a fixed wall (ensemble member).
+  * `dam_break_2d`      — 2D (per unit depth) Newtonian dam break in a walled box.
+  * `brazilian_disk_2d` — 2D Drucker-Prager disk with Grady-Kipp damage between two rigid
+                          platens.
 """
 from __future__ import annotations
 
@@ -40,6 +43,7 @@ class Scene:
     node_bc_type: np.ndarray = None  # int32[numNodes]
     node_bc_val: np.ndarray = None   # float64[numNodes, 4]
     meta: dict = field(default_factory=dict)
+    is3D: bool = True                # False: the plane path, L[2] = 0, one node layer
 
     # -- grid geometry (Nodes.cpp:12, 71-73) --
     @property
@@ -66,7 +70,8 @@ class Scene:
         p = GyParams()
         for k in range(3):
             p.gravity[k] = self.gravity[k]
-        p.damping, p.mu, p.cfl, p.psep, p.dt, p.is3D = self.damping, self.mu, self.cfl, self.psep, self.dt, 1
+        p.damping, p.mu, p.cfl, p.psep, p.dt = self.damping, self.mu, self.cfl, self.psep, self.dt
+        p.is3D = 1 if self.is3D else 0
         return p
 
     def resolved_dt(self) -> float:
@@ -271,4 +276,94 @@ def gk_disk(h=0.001, R=0.05, thickness=0.02, v=0.1, steps=2000) -> Scene:
     s.node_bc_type = np.zeros(s.num_nodes, np.int32)
     s.node_bc_val = np.zeros((s.num_nodes, 4))
     s.meta.update(disk=n1, platen=n2)
+    return s
+
+
+def plane_lattice(lo, counts, psep):
+    """(i+1/2)*psep lattice points in the z = 0 plane from corner `lo`, x fastest."""
+    nx, ny = counts
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    return np.stack([lo[0] + (i.ravel() + 0.5) * psep, lo[1] + (j.ravel() + 0.5) * psep,
+                     np.zeros(nx * ny)], axis=1)
+
+
+def dam_break_2d(h=0.02, L=1.0, steps=500, dt=None) -> Scene:
+    """2D dam break (after Example_Scripts/dambreak_2D.py): an L x L box (L[2] = 0), water
+    (model 2, K 1e6, rho 1000, viscosity 1) filling [2.75h, L/2] x [2.75h, 0.75L - 2h] at
+    psep = h/2, gravity -10 along y, damping 0.5, every node within 2h of a wall held by
+    grid_vx_vy [0, 0]. dt = 0.2 h / c (c = sqrt(K/rho)) unless given; the script's CFL of
+    1e-4 gives a dt 2000x smaller, which moves nothing in a parity run's few steps.
+    """
+    psep = h / 2
+    lo = 2.75 * h
+    counts = (int(round((0.5 * L - lo) / psep)), int(round((0.75 * L - 2 * h - lo) / psep)))
+    pos = plane_lattice((lo, lo), counts, psep)
+    K, rho = 1e6, 1000.0
+    water = material(model=2, bulk_mod=K, shear_mod=0.0, density=rho)
+    s = Scene("dam_break_2d", (L, L, 0.0), h, psep, [water], gravity=(0.0, -10.0, 0.0), damping=0.5,
+              dt=0.2 * h / math.sqrt(K / rho) if dt is None else dt, steps=steps, is3D=False)
+    s.particles = _particles(pos)
+    npos = s.node_positions()
+    wall = 2.0 * h
+    t = np.zeros(s.num_nodes, np.int32)
+    sel = (npos[:, 0] >= L - wall) | (npos[:, 0] <= wall) | (npos[:, 1] >= L - wall) | (npos[:, 1] <= wall)
+    t[sel] = BC["grid_vx_vy"]
+    s.node_bc_type = t
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    return s
+
+
+def brazilian_disk_2d(h=0.002, L=0.2, R=0.04, v=0.2, steps=300) -> Scene:
+    """2D Brazilian test (after Example_Scripts/brazillian2D.py): a disk of radius R
+    (object 1) of rock, Drucker-Prager (model 4: K 3.31e9, G 2.48e9, rho 1540, c 5.2e6,
+    phi = psi = 55.81, critical strain 0.1) with Grady-Kipp damage (crack speed 586.52,
+    m 9, k 7e35), squeezed by two rigid platens (objects 2 and 3, 6h wide, 2.5 mm thick)
+    moving at -v and +v along y. No gravity, friction 0, dt by the CFL rule (0.1)."""
+    psep = h / 2
+    c = 0.04 + R / 2.0
+    n = int(round(2 * R / psep))
+    box = plane_lattice((c - R, c - R), (n, n), psep)
+    disk = box[(box[:, 0] - c) ** 2 + (box[:, 1] - c) ** 2 <= R * R]
+    w, tp = 6 * h, 0.0025
+    cnt = (int(round(w / psep)), int(round(tp / psep)))
+    top = plane_lattice((c - w / 2, c + R), cnt, psep)
+    bot = plane_lattice((c - w / 2, c - R - tp), cnt, psep)
+    pos = np.concatenate([disk, top, bot])
+    n1, n2 = len(disk), len(top)
+    obj = np.concatenate([np.full(n1, 1), np.full(n2, 2), np.full(len(bot), 3)]).astype(np.int32)
+    mat = np.concatenate([np.zeros(n1), np.ones(2 * n2)]).astype(np.int32)
+    vel = np.zeros_like(pos)
+    vel[obj == 2, 1] = -v
+    vel[obj == 3, 1] = v
+    rock = material(model=4, bulk_mod=3.31e9, shear_mod=2.48e9, density=1540.0, cohesion=5.2e6,
+                    critical_strain=0.1, phi=55.81, psi=55.81, is_damage=True, damage_model=1,
+                    crack_speed=586.52, damage_m=9.0, damage_k=7.0e35)
+    platen = material(is_rigid=True)
+    s = Scene("brazilian_disk_2d", (L, L, 0.0), h, psep, [rock, platen], cfl=0.1, steps=steps, is3D=False)
+    s.particles = _particles(pos, vel, material=mat, obj=obj)
+    s.node_bc_type = np.zeros(s.num_nodes, np.int32)
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    s.meta.update(disk=n1, platen=n2)
+    return s
+
+
+def square_2d(h=0.02, L=0.5, side=0.2, vx=3.0, steps=60, mat=None) -> Scene:
+    """2D periodic plane box with a square of side `side` near the +x face moving at vx
+    with a shear v_y = 0.5 (x - x_c)/side, so it crosses the periodic wrap at L - 2h within
+    tens of steps. Elastic (K 1e6, G 5e5, rho 1000) unless `mat` is given; gravity -9.81
+    along y; dt = 0.2 h / c_p."""
+    psep = h / 2
+    n = int(round(side / psep))
+    lo = (L - 2 * h - side, (L - side) / 2)
+    pos = plane_lattice(lo, (n, n), psep)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = vx
+    vel[:, 1] = 0.5 * (pos[:, 0] - (lo[0] + side / 2)) / side
+    K, G, rho = 1e6, 5e5, 1000.0
+    m = material(model=1, bulk_mod=K, shear_mod=G, density=rho) if mat is None else mat
+    s = Scene("square_2d", (L, L, 0.0), h, psep, [m], gravity=(0.0, -9.81, 0.0),
+              dt=0.2 * h / math.sqrt((K + 4 * G / 3) / rho), steps=steps, is3D=False)
+    s.particles = _particles(pos, vel)
+    s.node_bc_type = np.zeros(s.num_nodes, np.int32)
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
     return s

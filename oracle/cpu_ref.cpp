@@ -10,10 +10,10 @@
 // tests/test_oracle_vs_reference.py against oracle/_ref). Every function cites the
 // reference lines it follows (paths relative to the Graphyt repository root).
 //
-// Scope: 3D only; constitutive models 1-5; Grady-Kipp damage (3D branch); one- and
-// two-body contact with rigid bodies; grid and particle BCs; periodic wrap. Thermal
-// fields are not simulated (T stays 0, as in every configuration studied here); the 2D
-// branches are out of scope.
+// Scope: 3D and 2D (Parameters::is3D, the reference's per-unit-depth plane path);
+// constitutive models 1-5; Grady-Kipp damage (both branches); one- and two-body contact
+// with rigid bodies; grid and particle BCs; periodic wrap. Thermal fields are not
+// simulated (T stays 0, as in every configuration studied here).
 //
 // Parity notes:
 //  * Grady-Kipp uses a cyclic Jacobi eigensolver (jacobi_eig3 below) instead of the
@@ -41,6 +41,7 @@ struct Sim {
   // ---- grid (Nodes.cpp:8-141) ----
   double L[3] = {0, 0, 0};
   double h = 0;
+  bool d3 = true;  // Parameters::is3D
   int Nx = 0, Ny = 0, Nz = 0, numNodes = 0;
   std::vector<double> npos;                // 3 per node
   std::vector<double> m, mb1, mb2;         // mass: COM, body 1, body 2
@@ -106,10 +107,15 @@ inline Shape shape(const Sim &s, int mp, int n) {
   Shape r;
   r.sx = gimp(s.x[3 * mp + 0], s.npos[3 * n + 0], s.h, lp, false);
   r.sy = gimp(s.x[3 * mp + 1], s.npos[3 * n + 1], s.h, lp, false);
-  r.sz = gimp(s.x[3 * mp + 2], s.npos[3 * n + 2], s.h, lp, false);
   r.gx = gimp(s.x[3 * mp + 0], s.npos[3 * n + 0], s.h, lp, true);
   r.gy = gimp(s.x[3 * mp + 1], s.npos[3 * n + 1], s.h, lp, true);
-  r.gz = gimp(s.x[3 * mp + 2], s.npos[3 * n + 2], s.h, lp, true);
+  if (s.d3) {
+    r.sz = gimp(s.x[3 * mp + 2], s.npos[3 * n + 2], s.h, lp, false);
+    r.gz = gimp(s.x[3 * mp + 2], s.npos[3 * n + 2], s.h, lp, true);
+  } else {  // per unit depth: shape_z = 1, gradShape_z = 0 (Extrapolation.cpp:184-188, 304-308)
+    r.sz = 1;
+    r.gz = 0.0;
+  }
   return r;
 }
 
@@ -133,7 +139,7 @@ void permute(std::vector<T> &v, const std::vector<int> &ord, int w) {
 }
 
 void sort_by_cell(Sim &s) {
-  const int ncell = (s.Nx - 1) * (s.Ny - 1) * (s.Nz - 1);
+  const int ncell = (s.Nx - 1) * (s.Ny - 1) * (s.d3 ? (s.Nz - 1) : 1);  // Extrapolation.cpp:372-378
   for (int c = 0; c < ncell; c++) s.cellCount[c] = 0;
   s.order.resize(s.np);
   for (int i = 0; i < s.np; i++) s.order[i] = i;
@@ -225,12 +231,23 @@ void p2g_pair(Sim &s, int mp, int n) {
 
 // Node-centric gather over a 4x4x4 window of cells — extrapToNodes, Extrapolation.cpp:31-89.
 // The window uses linear cell indices with only a range check (Extrapolation.cpp:61-66),
-// so at grid faces it also visits cells of the neighbouring row/slab.
+// so at grid faces it also visits cells of the neighbouring row/slab. 2D: a 4x4 window
+// checked against Cx*Cy (Extrapolation.cpp:89-121).
 void gather_node(Sim &s, int n) {
   const int Cx = s.Nx - 1, Cy = s.Ny - 1, Cz = s.Nz - 1;
   const int ix = (int)((s.npos[3 * n + 0] - s.npos[0]) / s.h);
   const int iy = (int)((s.npos[3 * n + 1] - s.npos[1]) / s.h);
   const int iz = (int)((s.npos[3 * n + 2] - s.npos[2]) / s.h);
+  if (!s.d3) {
+    for (int j = -2; j < 2; j++)
+      for (int i = -2; i < 2; i++) {
+        const int c = (ix + i) + (iy + j) * Cx;
+        if (c >= Cx * Cy || c < 0) continue;
+        const int first = s.cellStart[c], cnt = s.cellCount[c];
+        for (int mp = first; mp < first + cnt; mp++) p2g_pair(s, mp, n);
+      }
+    return;
+  }
   for (int k = -2; k < 2; k++)
     for (int j = -2; j < 2; j++)
       for (int i = -2; i < 2; i++) {
@@ -243,7 +260,8 @@ void gather_node(Sim &s, int n) {
 
 // ------------------------------------------------------------------------------------
 // Periodic pair sums — collectPeriodicContributions, Grid.cpp:310-603 (3D branch): node
-// (i) and node (N-3+i), i in {0,1,2}, both receive their sum; axes x, y, z in turn.
+// (i) and node (N-3+i), i in {0,1,2}, both receive their sum; axes x, y, z in turn. The 2D
+// branch (Grid.cpp:604-800) is the same over x and y.
 void pair_sum(Sim &s, int n, int q) {
   auto both = [&](std::vector<double> &arr, int w) {
     for (int k = 0; k < w; k++) {
@@ -269,6 +287,7 @@ void periodic(Sim &s) {
     for (int k = 0; k < Nz; k++)
       for (int i = 0; i < Nx; i++)
         pair_sum(s, i + j * Nx + k * Nx * Ny, i + (j + Ny - 3) * Nx + k * Nx * Ny);
+  if (!s.d3) return;
   for (int k = 0; k < 3; k++)
 #pragma omp parallel for
     for (int j = 0; j < Ny; j++)
@@ -404,8 +423,10 @@ void g2p(Sim &s, int mp) {
   const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
   const int bx = (int)((s.x[3 * mp + 0] - (s.psep / 2) - s.npos[0]) / s.h);
   const int by = (int)((s.x[3 * mp + 1] - (s.psep / 2) - s.npos[1]) / s.h);
-  const int bz = (int)((s.x[3 * mp + 2] - (s.psep / 2) - s.npos[2]) / s.h);
-  for (int k = 0; k < 3; k++)
+  // 2D: the 9 nodes of getGIMPNodes' plane branch (Extrapolation.cpp:683-705)
+  const int bz = s.d3 ? (int)((s.x[3 * mp + 2] - (s.psep / 2) - s.npos[2]) / s.h) : 0;
+  const int nk = s.d3 ? 3 : 1;  // Nz is 1 in 2D
+  for (int k = 0; k < nk; k++)
     for (int j = 0; j < 3; j++)
       for (int i = 0; i < 3; i++) {
         int n = (bx + i) + (by + j) * Nx + (bz + k) * Nx * Ny;
@@ -480,8 +501,8 @@ void elastic(const gy_material &M, const double *er, double *st, double dt) {
   st[5] += (2.0 * G * dev[5]) * dt;
 }
 
-// newtonianLiquid, ConstitutiveModels.cpp:33-64 (3D)
-void newtonian(const gy_material &M, const double *er, const double *eps, double *st, double dt) {
+// newtonianLiquid, ConstitutiveModels.cpp:33-64; 2D leaves sigma_zz, _yz, _zx as they are
+void newtonian(const gy_material &M, const double *er, const double *eps, double *st, double dt, bool d3) {
   const double K = M.bulk_mod, mu = M.viscosity;
   const double vol = 0.33333 * (er[0] + er[1] + er[2]);
   const double P = K * (eps[0] + eps[1] + eps[2]);
@@ -489,18 +510,19 @@ void newtonian(const gy_material &M, const double *er, const double *eps, double
   st[0] = P + mu * (2. - (2. / 3.)) * (dev[0]) * dt;
   st[1] = P + mu * (2. - (2. / 3.)) * (dev[1]) * dt;
   st[3] = 2.0 * mu * (dev[3]) * dt;
+  if (!d3) return;
   st[2] = P + mu * (2. - (2. / 3.)) * (dev[2]) * dt;
   st[4] = 2.0 * mu * (dev[4]) * dt;
   st[5] = 2.0 * mu * (dev[5]) * dt;
 }
 
-// turbulentLiquid, ConstitutiveModels.cpp:66-126 (3D). As there, the deviatoric rate is
+// turbulentLiquid, ConstitutiveModels.cpp:66-126 (2D as in newtonian). As there, the deviatoric rate is
 // formed while the volumetric part is still zero, so it is the full rate; the damage
 // field holds the turbulence flag: a set flag halves the viscosity of this update, then
 // the flag is set when KE - VE - turbStress > 0 and cleared (with the viscosity restored)
 // otherwise. KE uses the velocity and mass after this step's velocity update.
 void turbulent(const gy_material &M, const double *er, const double *eps, const double *vel, double mass,
-               double &dmg, double *st, double dt) {
+               double &dmg, double *st, double dt, bool d3) {
   const double K = M.bulk_mod, mu_mat = M.viscosity, turb = M.turb_stress;
   double mu = mu_mat;
   const double dev[6] = {er[0] - 0.0, er[1] - 0.0, er[2] - 0.0, er[3], er[4], er[5]};
@@ -517,23 +539,27 @@ void turbulent(const gy_material &M, const double *er, const double *eps, const 
   st[0] = P + mu * (2. - (2. / 3.)) * (dev[0]) * dt;
   st[1] = P + mu * (2. - (2. / 3.)) * (dev[1]) * dt;
   st[3] = 2.0 * mu * (dev[3]) * dt;
+  if (!d3) return;
   st[2] = P + mu * (2. - (2. / 3.)) * (dev[2]) * dt;
   st[4] = 2.0 * mu * (dev[4]) * dt;
   st[5] = 2.0 * mu * (dev[5]) * dt;
 }
 
-// perfectPlasticity, ConstitutiveModels.cpp:131-234 (3D)
+// perfectPlasticity, ConstitutiveModels.cpp:131-234. 2D: the trial sigma_zz, _yz, _zx are
+// zero (so the elastic branch writes zeros there) and the plastic branch leaves them.
 void perfect_plastic(const gy_material &M, const double *er, double *st, double *ep, double &epbar,
-                     double dt) {
+                     double dt, bool d3) {
   const double G = M.shear_mod, K = M.bulk_mod, sy = M.yield_stress;
   const double vol = 0.33 * (er[0] + er[1] + er[2]);
   double dev[6] = {er[0] - vol, er[1] - vol, er[2] - vol, er[3], er[4], er[5]};
-  double tr[6];
+  double tr[6] = {0, 0, 0, 0, 0, 0};
   tr[0] = st[0] + (2 * G * dev[0] + 3 * K * vol) * dt;
   tr[1] = st[1] + (2 * G * dev[1] + 3 * K * vol) * dt;
-  tr[2] = st[2] + (2 * G * dev[2] + 3 * K * vol) * dt;
-  tr[4] = st[4] + (2 * G * dev[4]) * dt;
-  tr[5] = st[5] + (2 * G * dev[5]) * dt;
+  if (d3) {
+    tr[2] = st[2] + (2 * G * dev[2] + 3 * K * vol) * dt;
+    tr[4] = st[4] + (2 * G * dev[4]) * dt;
+    tr[5] = st[5] + (2 * G * dev[5]) * dt;
+  }
   tr[3] = st[3] + (2 * G * dev[3]) * dt;
   const double q = std::sqrt(0.5 * std::pow((tr[0] - tr[1]), 2.0) + 0.5 * std::pow((tr[1] - tr[2]), 2.0) +
                              0.5 * std::pow((tr[2] - tr[0]), 2.0) + std::pow(tr[3], 2.0) +
@@ -561,6 +587,7 @@ void perfect_plastic(const gy_material &M, const double *er, double *st, double 
   st[0] += (2.0 * G * dev[0] + 3.0 * K * vol) * dt;
   st[1] += (2.0 * G * dev[1] + 3.0 * K * vol) * dt;
   st[3] += (2.0 * G * dev[3]) * dt;
+  if (!d3) return;
   st[2] += (2.0 * G * dev[2] + 3.0 * K * vol) * dt;
   st[4] += (2.0 * G * dev[4]) * dt;
   st[5] += (2.0 * G * dev[5]) * dt;
@@ -754,17 +781,56 @@ void grady_kipp(const gy_material &M, double *st, double &D, double &smax, doubl
   st[3] = out[0][1]; st[4] = out[1][2]; st[5] = out[0][2];
 }
 
+// gradyKippDamage 2D branch, DamageModels.cpp:11-69: principal stresses of the in-plane
+// tensor by the rotation angle, damage of the tensile ones, and sigma_zz set to 0 when the
+// damage grows or is complete. Outside those branches the stress is written back as read.
+void grady_kipp_2d(const gy_material &M, double *st, double &D, double &smax, double mass, double rho,
+                   double dt) {
+  const double Cg = M.crack_speed, m = M.damage_m, k = M.damage_k, G = M.shear_mod, K = M.bulk_mod;
+  double sxx = st[0], syy = st[1], sxy = st[3], szz = st[2];
+  const double theta = (std::fabs(sxx - syy) < 1e-5) ? 3.14159 / 4.0 : 0.5 * std::atan(2.0 * sxy / (sxx - syy));
+  const double c = std::cos(theta), sn = std::sin(theta);
+  double s1 = sxx * c * c + syy * sn * sn + 2.0 * c * sn * sxy;
+  double s2 = sxx * sn * sn + syy * c * c - 2.0 * c * sn * sxy;
+  const double smx = std::max(s1, std::max(s2, 0.0));
+  smax = smx;
+  const double alpha = (8.0 * 3.14159 * Cg * Cg * Cg * k) / (((m + 1.0) * (m + 2.0) * (m + 3.0)));
+  auto rebuild = [&]() {
+    if (s1 > 0.0) s1 = (1.0 - D) * s1;
+    if (s2 > 0.0) s2 = (1.0 - D) * s2;
+    sxx = s1 * c * c + s2 * sn * sn;
+    syy = s1 * sn * sn + s2 * c * c;
+    sxy = (s1 - s2) * c * sn;
+    szz = 0.0;
+  };
+  if (smx > 1e-3) {
+    const double eff = smx / (K + (4.0 * G / 3.0));
+    const double vol = mass / rho;
+    const double emin = std::pow(vol * k, -1.0 / m);
+    if (eff > emin) {
+      const double rate = ((m + 3.0) / 3.0) * std::pow(alpha, 1.0 / 3.0) * std::pow(eff, m / 3.0);
+      D = std::pow(std::pow(D, 1.0 / 3.0) + (dt * rate), 3.0);
+      if (D > 1.0) D = 1.0;
+      rebuild();
+    }
+  }
+  if (D == 1.0) rebuild();
+  st[0] = sxx; st[1] = syy; st[2] = szz; st[3] = sxy;
+}
+
 // ------------------------------------------------------------------------------------
-// Particle update — updateMatPoint, TimeIntegration.cpp:11-117 (3D).
+// Particle update — updateMatPoint, TimeIntegration.cpp:11-117. 2D: v_z and x_z are not
+// advanced and the periodic wrap acts on x and y only.
 void update_particle(Sim &s, int mp) {
   const double dt = s.dt;
   const double Lp[3] = {s.L[0] - 2 * s.h, s.L[1] - 2 * s.h, s.L[2] - 2 * s.h};
   s.rho[mp] += s.drate[mp] * dt;
   s.vol[mp] = s.mass[mp] / s.rho[mp];
+  const int dim = s.d3 ? 3 : 2;
   if (!s.hold[mp]) {
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < dim; k++)
       s.vel[3 * mp + k] = s.vel[3 * mp + k] + (s.acc[3 * mp + k] - s.damping * s.vel[3 * mp + k]) * dt;
-    for (int k = 0; k < 3; k++) s.x[3 * mp + k] += s.gvel[3 * mp + k] * dt;
+    for (int k = 0; k < dim; k++) s.x[3 * mp + k] += s.gvel[3 * mp + k] * dt;
   }
   for (int k = 0; k < 3; k++) s.u[3 * mp + k] = s.x[3 * mp + k] - s.x0[3 * mp + k];
   for (int k = 0; k < 6; k++) s.eps[6 * mp + k] += s.erate[6 * mp + k] * dt;
@@ -774,17 +840,19 @@ void update_particle(Sim &s, int mp) {
     const double *er = &s.erate[6 * mp];
     switch (M.model) {
       case 1: elastic(M, er, st, dt); break;
-      case 2: newtonian(M, er, &s.eps[6 * mp], st, dt); break;
-      case 3: perfect_plastic(M, er, st, &s.epsp[6 * mp], s.epbar[mp], dt); break;
+      case 2: newtonian(M, er, &s.eps[6 * mp], st, dt, s.d3); break;
+      case 3: perfect_plastic(M, er, st, &s.epsp[6 * mp], s.epbar[mp], dt, s.d3); break;
       case 4: drucker_prager(M, er, st, &s.epsp[6 * mp], s.epbar[mp], dt); break;
-      case 5: turbulent(M, er, &s.eps[6 * mp], &s.vel[3 * mp], s.mass[mp], s.dmg[mp], st, dt); break;
+      case 5: turbulent(M, er, &s.eps[6 * mp], &s.vel[3 * mp], s.mass[mp], s.dmg[mp], st, dt, s.d3); break;
       default: break;
     }
-    if (M.is_damage && M.damage_model == 1)
-      grady_kipp(M, st, s.dmg[mp], s.smax[mp], s.mass[mp], s.rho[mp], dt);
+    if (M.is_damage && M.damage_model == 1) {
+      if (s.d3) grady_kipp(M, st, s.dmg[mp], s.smax[mp], s.mass[mp], s.rho[mp], dt);
+      else grady_kipp_2d(M, st, s.dmg[mp], s.smax[mp], s.mass[mp], s.rho[mp], dt);
+    }
   }
   // periodic particle wrap (TimeIntegration.cpp:80-108)
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < dim; k++) {
     double &xk = s.x[3 * mp + k];
     if (xk > Lp[k]) xk = 1.0 * s.h + std::fmod((xk + Lp[k]), Lp[k]);
     if (xk < 1 * s.h) xk += Lp[k] - 1.0 * s.h;
@@ -802,7 +870,11 @@ void update_particle(Sim &s, int mp) {
 void iterate(Sim &s, double t, int step) {
   if ((step % 10 == 0) || (step == 1)) sort_by_cell(s);
   const int N = s.numNodes, P = s.np;
-  const int nthr = 1;  // chunk expression evaluated outside the team (Solver.cpp:120-249)
+  // The reference's chunk N/(omp_get_num_threads()*16) (Solver.cpp:120-249) is evaluated
+  // inside the team, so it is 0 (libgomp then never advances a dynamic loop: the step
+  // hangs) when there are fewer than 16 items per thread; here it is kept at least 1.
+  // The chunk decides which thread runs an item, never a result.
+  const int nthr = std::max(1, omp_get_max_threads());
   const int chN = std::max(1, N / (nthr * 16)), chP = std::max(1, P / (nthr * 16));
 #pragma omp parallel for
   for (int n = 0; n < N; n++) reset_node(s, n);
@@ -837,6 +909,7 @@ Sim *create(const gy_grid *gr, const gy_material *mats, int nmat, const gy_param
   Sim *s = new Sim();
   for (int k = 0; k < 3; k++) s->L[k] = gr->L[k];
   s->h = gr->cellsize;
+  s->d3 = pr->is3D != 0;
   s->numNodes = (int)std::round((1 + (s->L[0] / s->h)) * (1 + (s->L[1] / s->h)) * (1 + (s->L[2] / s->h)));
   s->Nz = (int)(1 + (s->L[2] / s->h));
   s->Ny = (int)(1 + (s->L[1] / s->h));
@@ -908,16 +981,17 @@ Sim *create(const gy_grid *gr, const gy_material *mats, int nmat, const gy_param
   }
   s->dt = pr->cfl * (s->h / std::sqrt(maxK / minrho));
   if (pr->dt > 0) s->dt = pr->dt;
-  // Solver::initialize (Solver.cpp:42-56)
+  // Solver::initialize (Solver.cpp:40-67): per unit depth in 2D, where rigid particles
+  // keep rho*V (P2G still weighs them with unit mass)
   s->mass.assign(P, 0.0);
   s->rho.assign(P, 0.0);
   s->vol.assign(P, 0.0);
   for (int i = 0; i < P; i++) {
     const gy_material &M = s->mats[s->mat[i]];
     s->rho[i] = M.density;
-    s->vol[i] = s->psep * s->psep * s->psep;
+    s->vol[i] = s->d3 ? s->psep * s->psep * s->psep : s->psep * s->psep;
     s->mass[i] = s->rho[i] * s->vol[i];
-    if (M.is_rigid) s->mass[i] = 1.0;
+    if (M.is_rigid && s->d3) s->mass[i] = 1.0;
   }
   return s;
 }
@@ -1010,8 +1084,8 @@ void cref_constitutive(const gy_material *M, const double *er, const double *eps
                        double *ep, double *epbar, double dt) {
   switch (M->model) {
     case 1: elastic(*M, er, st, dt); break;
-    case 2: newtonian(*M, er, eps, st, dt); break;
-    case 3: perfect_plastic(*M, er, st, ep, *epbar, dt); break;
+    case 2: newtonian(*M, er, eps, st, dt, true); break;
+    case 3: perfect_plastic(*M, er, st, ep, *epbar, dt, true); break;
     case 4: drucker_prager(*M, er, st, ep, *epbar, dt); break;
     default: break;
   }

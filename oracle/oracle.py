@@ -85,6 +85,14 @@ class CpuSim:
 
     def __init__(self, scene, kind: str = "restatement"):
         self.kind = kind
+        if kind == "reference":
+            # The reference's dynamic OpenMP chunk, P/(16*threads), is evaluated inside the
+            # team (Solver.cpp:120-249): below 16 particles per thread it is 0 and libgomp
+            # never advances the loop, so the step would hang.
+            threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+            if scene.n < 16 * threads:
+                raise ValueError(f"{scene.n} particles < 16 x {threads} OpenMP threads: the reference's "
+                                 "chunk would be 0 and its step would hang (set OMP_NUM_THREADS lower)")
         self.lib, self.pre = _lib(kind)
         self.scene = scene
         self._mats = material_array(scene.mats)

@@ -39,7 +39,7 @@ def main(names=None):
     oracle.build(reference=True)
     if not oracle.available("reference"):
         raise SystemExit("the reference library could not be built (no reference sources here)")
-    for name in names or scenes.NAMES:
+    for name in names or scenes.ALL_NAMES:
         sc, steps, t_of = scenes.build(name)
         sim = oracle.CpuSim(sc, "reference")
         run(sim, steps, t_of)

@@ -15,6 +15,17 @@ a few dozen steps each), one per part of the step the fixtures pin:
   rigid_platen     deformable block pressed by a rigid platen (two objects)
   two_blocks_mu    two deformable halves in contact, sliding, friction coefficient 0.3
 
+and the plane (2D, Parameters is3D = False) path, per unit depth:
+
+  plane_dam_break       Newtonian water column in a walled box (dambreak_2D.py)
+  plane_elastic         elastic square crossing the periodic wrap, shear velocity field
+  plane_perfect_plastic the same square, von Mises (model 3) yielding everywhere
+  plane_turbulent       the same square as a turbulent liquid: the flag set on a subset
+  plane_grady_kipp      the same square with Grady-Kipp damage (closed-form 2D branch),
+                        partly and fully damaged particles
+  plane_brazilian       Drucker-Prager disk with damage between two rigid platens
+                        (brazillian2D.py), three objects
+
 `build(name)` returns (scene, steps, t_of); t_of(step) is the time passed to each
 iterate call, or None for step * dt.
 """
@@ -27,6 +38,9 @@ from graphyt_amd.abi import BC, material
 
 NAMES = ["elastic", "newtonian", "turbulent", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
          "bcs", "rigid_platen", "two_blocks_mu"]
+PLANE_NAMES = ["plane_dam_break", "plane_elastic", "plane_perfect_plastic", "plane_turbulent",
+               "plane_grady_kipp", "plane_brazilian"]
+ALL_NAMES = NAMES + PLANE_NAMES
 
 # fields stored per scene (FP64) and compared
 BASE_FIELDS = ["pos", "vel", "stress", "strain", "density", "cell_id"]
@@ -36,6 +50,10 @@ EXTRA_FIELDS = {
     "dp_unguarded": ["plastic_strain", "acc_plastic_strain"],
     "grady_kipp": ["damage", "smax"],
     "turbulent": ["damage"],
+    "plane_perfect_plastic": ["plastic_strain", "acc_plastic_strain"],
+    "plane_turbulent": ["damage"],
+    "plane_grady_kipp": ["damage", "smax"],
+    "plane_brazilian": ["plastic_strain", "acc_plastic_strain", "damage", "smax"],
 }
 
 
@@ -126,6 +144,22 @@ def build(name: str):
         vel[left, 1], vel[~left, 1] = 0.2, -0.2
         sc.particles.arrays.update(obj_id=np.where(left, 1, 2).astype(np.int32), vel=vel)
         return sc, 60, None
+    if name == "plane_dam_break":
+        return W.dam_break_2d(steps=60), 60, None
+    if name == "plane_elastic":
+        return W.square_2d(steps=60), 60, None
+    if name == "plane_perfect_plastic":
+        m = material(model=3, bulk_mod=1e6, shear_mod=5e5, density=1000.0, yield_stress=300.0)
+        return W.square_2d(steps=60, mat=m), 60, None
+    if name == "plane_turbulent":
+        m = material(model=5, bulk_mod=1e6, viscosity=1e-3, density=1000.0, turb_stress=0.452)
+        return W.square_2d(steps=60, mat=m), 60, None
+    if name == "plane_grady_kipp":
+        m = material(model=1, bulk_mod=1e6, shear_mod=5e5, density=1000.0, is_damage=True, damage_model=1,
+                     crack_speed=0.5, damage_m=9.0, damage_k=1e40)
+        return W.square_2d(steps=60, mat=m), 60, None
+    if name == "plane_brazilian":
+        return W.brazilian_disk_2d(h=0.004, v=20.0, steps=300), 300, None
     raise KeyError(name)
 
 

@@ -29,7 +29,7 @@ def run(sim, steps, t_of):
             sim.iterate(t_of(s), s)
 
 
-@pytest.mark.parametrize("name", scenes.NAMES)
+@pytest.mark.parametrize("name", scenes.ALL_NAMES)
 def test_restatement_matches_reference_fixture(oracle_mod, name):
     fx = load(name)
     sc, steps, t_of = scenes.build(name)
@@ -65,3 +65,23 @@ def test_fixtures_exercise_their_branch():
     assert np.array_equal(load("bcs")["pos"][held], sc.particles["pos"][held])
     v = load("two_blocks_mu")["vel"]
     assert np.abs(v[:, 0]).mean() < 0.35  # the halves (0.5 m/s each) slowed each other down
+
+
+def test_plane_fixtures_exercise_their_branch():
+    """The plane scenes: z never advances, the wrap acts on x, and each material branch is
+    reached (yield, the turbulence flag on a subset, partial and full damage, plastic flow
+    and damage in the disk)."""
+    for name in scenes.PLANE_NAMES:
+        sc, _, _ = scenes.build(name)
+        fx = load(name)
+        assert np.array_equal(fx["pos"][:, 2], sc.particles["pos"][:, 2]), name
+        assert np.all(fx["stress"][:, 4:] == 0.0) or name == "plane_brazilian", name
+    x0 = scenes.build("plane_elastic")[0].particles["pos"][:, 0]
+    assert (load("plane_elastic")["pos"][:, 0] < x0 - 0.1).sum() > 10  # wrapped to the -x side
+    assert (load("plane_perfect_plastic")["acc_plastic_strain"] > 0).sum() > 10
+    flag = load("plane_turbulent")["damage"]
+    assert 0 < (flag == 1.0).sum() < flag.size
+    d = load("plane_grady_kipp")["damage"]
+    assert ((d > 0) & (d < 1)).sum() > 10 and (d == 1).sum() > 10
+    br = load("plane_brazilian")
+    assert (br["damage"] > 0).sum() > 100 and (br["acc_plastic_strain"] > 0).sum() > 100

@@ -170,3 +170,24 @@ def test_eig3_against_reference(ref):
         assert np.allclose(da, db, rtol=1e-13, atol=1e-13 * np.abs(db).max())
         # same spectral reconstruction
         assert np.allclose(Va @ np.diag(da) @ Va.T, Vb @ np.diag(db) @ Vb.T, atol=1e-12)
+
+
+def test_plane_dam_break_bit_identical(ref):
+    """The plane path (is3D = False): 9-node stencil with shape_z = 1, 4x4 gather window,
+    x/y periodic pair sums, per-unit-depth mass; node fields too."""
+    ra, rb, a, b = _pair(ref, W.dam_break_2d(steps=40), 40)
+    _assert_identical(ra, rb)
+    na, nb = a.nodes(), b.nodes()
+    for k in na:
+        assert np.array_equal(na[k], nb[k]), k
+    assert np.all(ra["pos"][:, 2] == 0.0) and np.abs(ra["vel"][:, 1]).max() > 0
+
+
+def test_plane_brazilian_disk_bit_identical(ref):
+    """Plane Drucker-Prager with the closed-form 2D Grady-Kipp branch and rigid platens;
+    unlike the 3D eigensolver path this one is bit for bit."""
+    sc = W.brazilian_disk_2d(h=0.004, v=20.0)
+    ra, rb, _, _ = _pair(ref, sc, 200)
+    _assert_identical(ra, rb)
+    assert (ra["damage"] > 0).sum() > 100
+
