@@ -50,6 +50,7 @@ struct gy_ctx {
   std::vector<MatDev> mats;
   DevPtrs d{};
   long long nnodes = 0;
+  bool plane = false;  // Parameters::is3D false for every instance
   bool uploaded = false;
   bool acc_valid = false;
   int host_step = 0;
@@ -186,11 +187,15 @@ MatDev make_mat(const gy_material &m) {
 int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, const gy_params *p,
              InstHost &H, InstDesc &I) {
   if (!g || !mats || !p || nmat <= 0) return fail(c, GY_ERR_INVALID, "null grid/materials/params");
-  if (!p->is3D) return fail(c, GY_ERR_UNSUPPORTED, "only the 3D path is implemented");
   if (nmat > 16) return fail(c, GY_ERR_UNSUPPORTED, "at most 16 materials per instance");
   const double h = g->cellsize;
   if (!(h > 0)) return fail(c, GY_ERR_INVALID, "cellsize must be positive");
-  for (int a = 0; a < 3; a++) {
+  // plane path: one node layer, Nz = 1 + L[2]/h = 1 (Nodes.cpp:71-73 with L[2] = 0)
+  const bool plane = !p->is3D;
+  if (plane && g->L[2] != 0.0) return fail(c, GY_ERR_INVALID, "a plane (is3D = false) grid needs L[2] = 0");
+  I.plane = plane ? 1 : 0;
+  I.n[2] = 1;
+  for (int a = 0; a < (plane ? 2 : 3); a++) {
     const double r = g->L[a] / h;
     if (std::fabs(r - std::round(r)) > 1e-9 * std::max(1.0, r))
       return fail(c, GY_ERR_INVALID, "L[%d]/cellsize = %.12g is not an integer (Nodes.cpp:12 vs 71)", a, r);
@@ -223,11 +228,19 @@ int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, con
   I.inv_lp = (float)(h / (0.5 * p->psep));
   I.lpq = (int)llround(0.5 * p->psep / h * (double)(1 << F));
   for (int a = 0; a < 3; a++) {
-    I.cper[a] = I.n[a] - 3;
-    I.nt[a] = (I.n[a] - 2 + TILE - 1) / TILE;
-    I.nb[a] = (I.cper[a] + BRICK - 1) / BRICK;
-    I.wrapq[a] = (int)llround((g->L[a] - 2 * h) / h * (double)(1 << F));
     I.gravity[a] = (float)p->gravity[a];
+    if (plane && a == 2) {  // one node layer: no z tiles, bricks, pairs or wrap
+      I.cper[a] = 1;
+      I.nt[a] = 1;
+      I.nb[a] = 1;
+      I.wrapq[a] = 0;
+      continue;
+    }
+    const int tile = plane ? PT : TILE, brick = plane ? PB : BRICK;
+    I.cper[a] = I.n[a] - 3;
+    I.nt[a] = (I.n[a] - 2 + tile - 1) / tile;
+    I.nb[a] = (I.cper[a] + brick - 1) / brick;
+    I.wrapq[a] = (int)llround((g->L[a] - 2 * h) / h * (double)(1 << F));
   }
   I.h = (float)h;
   I.inv_h = (float)(1.0 / h);
@@ -322,6 +335,14 @@ int check_status(gy_ctx *c) {
 // One step after the binning of the current buffer is in place.
 void enqueue_step(gy_ctx *c) {
   void *s = c->stream;
+  if (c->plane) {
+    timed(c, K_GRID, [&] { launch_plane_grid(c->d, s); });
+    timed(c, K_G2P2G, [&] { launch_plane_g2p2g(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
+    timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    return;
+  }
   if (c->d.multibody) {
     timed(c, K_GRID, [&] { launch_mb_grid(c->d, s); });
     timed(c, K_G2P2G, [&] { launch_mb_update(c->d, s); });
@@ -343,6 +364,15 @@ void enqueue_step(gy_ctx *c) {
 // First step after an upload: bin the uploaded state and scatter it to the grid.
 void enqueue_init(gy_ctx *c) {
   void *s = c->stream;
+  if (c->plane) {
+    timed(c, K_KEYS, [&] { launch_plane_keys(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
+    timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
+    launch_brick_list(c->d, s);
+    return;
+  }
   timed(c, K_KEYS, [&] { launch_keys(c->d, s); });
   timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
   timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
@@ -412,7 +442,10 @@ static int create_impl(gy_ctx **out, int32_t ninst, const gy_grid *grids, const 
       return rc;
     }
     moff += nmat[i];
+    if (c->inst[i].plane != c->inst[0].plane)
+      return fail(c, GY_ERR_UNSUPPORTED, "instance %d: a context holds plane (is3D = false) or 3D instances, not both", i);
   }
+  c->plane = c->inst[0].plane != 0;
   return alloc_grid(c);
 }
 
@@ -515,6 +548,7 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     H.carry_damage.assign(nmat, 0);
     H.pos0.resize(3 * (size_t)n);
     const double psep3 = H.params.psep * H.params.psep * H.params.psep;
+    const double psep2 = H.params.psep * H.params.psep;
     for (int i = 0; i < n; i++) {
       const long long s_ = pstart + i;
       for (int a = 0; a < 3; a++) {
@@ -538,9 +572,11 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
         put(F_EPB, s_, in->acc_plastic_strain[i]);
         if (in->acc_plastic_strain[i] != 0.0) H.carry_plastic[m] = 1;
       }
-      // Solver::initialize (Solver.cpp:42-56) when density/mass are not given
+      // Solver::initialize (Solver.cpp:40-67) when density/mass are not given: per unit
+      // depth in a plane, where rigid particles keep rho*V (P2G weighs them with unit mass)
       const double rho = in->density ? in->density[i] : H.mats[m].density;
-      const double mass = in->mass ? in->mass[i] : (H.mats[m].is_rigid ? 1.0 : rho * psep3);
+      const double mass = in->mass ? in->mass[i]
+                                   : I.plane ? rho * psep2 : (H.mats[m].is_rigid ? 1.0 : rho * psep3);
       put(F_RHO, s_, rho);
       put(F_MASS, s_, mass);
       if (in->damage) { put(F_DMG, s_, in->damage[i]); if (in->damage[i] != 0.0) H.carry_damage[m] = 1; }
@@ -577,6 +613,8 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     I.pcount = n;
     pstart += n;
   }
+  if (multibody && c->plane)
+    return fail(c, GY_ERR_UNSUPPORTED, "plane (is3D = false) contexts hold one deformable object (no contact yet)");
   d.kmask = kmask;
   d.multibody = multibody ? 1 : 0;
   if (multibody) {

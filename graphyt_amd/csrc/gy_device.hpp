@@ -1,0 +1,183 @@
+// gy_device.hpp — device helpers shared by the step kernels of the 3D path
+// (gy_kernels.hip) and the plane path (gy_plane.hip): launch constants, fixed-point
+// stencil bases, the GIMP weights of one axis, particle BC handlers, the P2G terms of one
+// node, the histogram rank of a wave's keys and the active-tile work queue.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gy_internal.hpp"
+
+namespace gy {
+namespace {
+
+constexpr int WG = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
+constexpr int MAXMAT = 16;
+constexpr int CHUNK_ENTRIES = WG;  // list entries per chunk of the fused particle kernel
+
+__device__ __forceinline__ float f32(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
+
+// Stencil base cell of a fixed-point coordinate, clamped to the grid (getGIMPNodes,
+// Extrapolation.cpp:658-660: int((x - psep/2 - x0)/h)); `bad` is set when clamped.
+__device__ __forceinline__ int stencil_base(int q, int lpq, int F, int maxb, bool &bad) {
+  int b = (q - lpq) >> F;
+  if (b < 0) { b = 0; bad = true; }
+  if (b > maxb) { b = maxb; bad = true; }
+  return b;
+}
+
+__device__ __forceinline__ int key_of(const InstDesc &I, const int b[3]) {
+  const int t = I.tile_base + (b[0] / TILE) + I.nt[0] * ((b[1] / TILE) + I.nt[1] * (b[2] / TILE));
+  return t * TILE3 + (b[0] % TILE) + TILE * ((b[1] % TILE) + TILE * (b[2] % TILE));
+}
+
+// GIMP weights of the three stencil nodes along one axis (Extrapolation.cpp:556-647 with
+// h = 1), for a particle at offset u in [lp, 1+lp) from the base node and lp <= 1/2.
+// S[i] = S(d_i), G[i] = dS/dd at d_i = i - u; the physical gradient dN/dx_p is -G/h.
+struct Axis {
+  float w[3], g[3];
+};
+// one of three values by a loop index k in {0,1,2}, by value so no array is spilled
+__device__ __forceinline__ float sel3(int k, float a0, float a1, float a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
+__device__ __forceinline__ Axis gimp3(float u, float lp, float inv_lp, float inv_h) {
+  Axis A;
+  const float d0 = -u, d1 = 1.0f - u, d2 = 2.0f - u;
+  const float e0 = 1.0f + lp + d0;  // distance past -(1+lp)
+  // node 0: d0 in (-1-lp, -lp]
+  if (d0 <= -1.0f + lp) { A.w[0] = e0 * e0 * (0.25f * inv_lp); } else { A.w[0] = 1.0f + d0; }
+  const float G0 = (d0 < -1.0f + lp) ? e0 * (0.5f * inv_lp) : 1.0f;
+  // node 1: d1 in (-lp, 1-lp]
+  if (d1 <= lp) { A.w[1] = 1.0f - (d1 * d1 + lp * lp) * (0.5f * inv_lp); } else { A.w[1] = 1.0f - d1; }
+  const float G1 = (d1 < lp) ? -d1 * inv_lp : -1.0f;
+  // node 2: d2 in (1-lp, 2-lp]
+  const float e2 = 1.0f + lp - d2;
+  float G2;
+  if (d2 <= 1.0f + lp) { A.w[2] = e2 * e2 * (0.25f * inv_lp); G2 = -e2 * (0.5f * inv_lp); }
+  else { A.w[2] = 0.0f; G2 = 0.0f; }
+  A.g[0] = -G0 * inv_h;
+  A.g[1] = -G1 * inv_h;
+  A.g[2] = -G2 * inv_h;
+  return A;
+}
+
+// Particle BC handlers, TimeIntegration.cpp:177-296 (phase is ignored by the reference).
+__device__ __forceinline__ void particle_bc(int type, const float *bv, float t, float v[3], float a[3],
+                                            float vg[3], float s[6], float mass) {
+  switch (type) {
+    case 33: v[0] = bv[0]; break;
+    case 34: v[0] = 0.0f; v[1] = bv[0]; v[2] = 0.0f; break;
+    case 35: v[0] = 0.0f; v[1] = 0.0f; v[2] = bv[0]; break;
+    case 40: v[0] = bv[0]; v[1] = bv[1]; v[2] = 0.0f; break;
+    case 65: v[0] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t); break;
+    case 66: v[1] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t); vg[1] = 0.0f; a[1] = 0.0f; break;
+    case 67:
+      s[1] = bv[0] * sinf(2.0f * 3.14159f * bv[1] * t);
+      v[0] = v[1] = v[2] = 0.0f; vg[0] = vg[1] = vg[2] = 0.0f; break;
+    case 37: a[0] += bv[0] / mass; break;
+    case 38: a[0] = 0.0f; a[1] += bv[0] / mass; break;
+    case 39: a[2] += bv[0] / mass; break;
+    case 43: a[0] += bv[0] / mass; a[1] += bv[1] / mass; break;
+    case 63: s[0] = bv[0]; s[1] = bv[1]; break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ int inst_of_slot(const DevPtrs &d, int s) {
+  int lo = 0, hi = d.ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.inst[mid].pstart <= s) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Scatter one particle's P2G terms of one node: m, m*v, and the internal force
+// -V * sigma . gradN (Extrapolation.cpp:265-273).
+struct P2GTerms {
+  float m, mv[3], vs[6];  // vs = -V*sigma
+};
+
+__device__ __forceinline__ void node_terms(const P2GTerms &T, float w, float gx, float gy, float gz,
+                                           float out[7]) {
+  out[0] = w * T.m;
+  out[1] = w * T.mv[0];
+  out[2] = w * T.mv[1];
+  out[3] = w * T.mv[2];
+  out[4] = T.vs[0] * gx + T.vs[3] * gy + T.vs[5] * gz;
+  out[5] = T.vs[3] * gx + T.vs[1] * gy + T.vs[4] * gz;
+  out[6] = T.vs[5] * gx + T.vs[4] * gy + T.vs[2] * gz;
+}
+
+// Model-set bits: the fused kernel is instantiated per set of constitutive models a
+// context uses, so a single-material run carries only its own model's registers.
+constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32,
+              KM_TURB = 64;
+constexpr int KM_ALL = 127;
+
+// Rank of a lane's key among the equal keys of its wave, with one global atomic per run
+// of equal keys (lanes hold consecutive entries of a cell-grouped list, so equal keys
+// are adjacent). key < 0: the lane takes no part (only trailing lanes may do so).
+__device__ __forceinline__ int wave_hist_rank(int *count, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = key >= 0 && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valid = __ballot(key >= 0);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  int base = 0;
+  if (head) {
+    const unsigned long long later = heads & ~upto;
+    const int vend = 64 - __clzll((long long)valid);
+    const int end = later ? __ffsll((long long)later) - 1 : vend;
+    base = atomicAdd(&count[key], end - lane);
+  }
+  const unsigned long long mine = heads & upto;
+  const int hl = mine ? 63 - __clzll((long long)mine) : lane;
+  base = __shfl(base, hl, 64);
+  return base + lane - hl;
+}
+
+// The same rank in two halves, so the atomic's round trip can overlap other work:
+// hist_issue adds the run lengths (head lanes get the old counts in `base`), hist_rank
+// then spreads each run's base over its lanes.
+__device__ __forceinline__ int hist_issue(int *count, int key, int &hl) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = key >= 0 && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valid = __ballot(key >= 0);
+  const unsigned long long upto = (2ull << lane) - 1ull;
+  int base = 0;
+  if (head) {
+    const unsigned long long later = heads & ~upto;
+    const int vend = 64 - __clzll((long long)valid);
+    const int end = later ? __ffsll((long long)later) - 1 : vend;
+    base = atomicAdd(&count[key], end - lane);
+  }
+  const unsigned long long mine = heads & upto;
+  hl = mine ? 63 - __clzll((long long)mine) : lane;
+  return base;
+}
+__device__ __forceinline__ int hist_rank(int base, int hl) {
+  return __shfl(base, hl, 64) + (int)(threadIdx.x & 63) - hl;
+}
+
+// The particle kernels' work queue over the active tiles: heavy tiles first (in tile
+// order), then the light ones, so the claims that end a launch are the short ones; one
+// counter for the whole device (balance across the XCDs matters more here than keeping
+// neighbouring tiles in one L2: measured). Bounds read once per launch.
+struct TileQueue {
+  int nh, n;
+  __device__ __forceinline__ explicit TileQueue(const DevPtrs &d) : nh(d.ctrl->heavy_tiles), n(d.ctrl->active_tiles) {}
+  // tile of claim k, -1 past the end
+  __device__ __forceinline__ int tile(const DevPtrs &d, int k) const {
+    if (k < nh) return d.active[k];
+    if (k < n) return d.active[n - 1 - (k - nh)];
+    return -1;
+  }
+};
+
+}  // namespace
+}  // namespace gy

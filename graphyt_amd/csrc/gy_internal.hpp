@@ -14,6 +14,10 @@
 //
 // Binning: cells are numbered tile-major (tiles of TILE^3 cells, global tile id across
 // all instances), so a counting sort by cell also groups particles by tile.
+//
+// Plane instances (Parameters::is3D false, one node layer): tiles of PT x PT cells (the
+// same TILE3 keys per tile), bricks of PB x PB canonical nodes (one wave), and the same
+// per-tile partial buffers holding the smaller plane regions.
 #pragma once
 #include <cstdint>
 
@@ -32,6 +36,18 @@ constexpr int PARTIAL_W = 8 * P2G_NODES;   // floats of one tile's full-region p
 constexpr int CORE_R = TILE + 2;           // edge of the region a tile's own cells reach
 constexpr int CORE_NODES = CORE_R * CORE_R * CORE_R;
 constexpr int CORE_W = 8 * CORE_NODES;     // floats of one tile's core partial sums
+
+// plane path
+constexpr int PT = 8;                      // cells per plane tile edge
+constexpr int PB = 8;                      // canonical nodes per plane brick edge
+constexpr int PG_R = PT + 2;               // staged VEL region edge: nodes [o, o+PT+2)
+constexpr int PR = PT + 4;                 // LDS P2G region edge: nodes [o-1, o+PT+3)
+constexpr int PR_N = PR * PR;
+constexpr int PC = PT + 2;                 // core edge: region coordinates 1..PT+2
+constexpr int PC_N = PC * PC;
+static_assert(PT * PT == TILE3, "a plane tile has the keys of a 3D tile");
+static_assert(PB * PB == 64, "a plane brick is one wave");
+static_assert(8 * PC_N <= CORE_W && 8 * PR_N <= PARTIAL_W && PR_N <= P2G_NODES, "plane regions fit the tile buffers");
 
 enum PField : int {
   F_QX = 0, F_QY, F_QZ,                     // fixed-point position (int32 bits)
@@ -84,6 +100,7 @@ struct InstDesc {
   float damping;
   float inv_2F;      // 2^-F
   float mu;          // contact friction coefficient (Parameters::mu)
+  int plane;         // Parameters::is3D false: n[2] = 1, plane tiles and bricks
 };
 
 // Per-material constants, derived on the host in FP64 (ConstitutiveModels.cpp,
@@ -208,5 +225,11 @@ void launch_mb_update(const DevPtrs &d, void *stream);
 void launch_mb_claim(const DevPtrs &d, void *stream);   // cell objects + node body claims
 void launch_mb_p2g(const DevPtrs &d, void *stream);
 int scan_items_per_block();
+// plane path (gy_plane.hip): keys, P2G of the uploaded state, the fused particle kernel and
+// the grid kernel of plane instances; scan, place and brick list are shared
+void launch_plane_keys(const DevPtrs &d, void *stream);
+void launch_plane_p2g_only(const DevPtrs &d, void *stream);
+void launch_plane_g2p2g(const DevPtrs &d, void *stream);
+void launch_plane_grid(const DevPtrs &d, void *stream);
 
 }  // namespace gy

@@ -27,7 +27,9 @@ __device__ __forceinline__ void cm_elastic(const MatDev &M, const float er[6], f
   s[5] += (G2 * er[5]) * dt;
 }
 
-// newtonianLiquid, ConstitutiveModels.cpp:33-64
+// newtonianLiquid, ConstitutiveModels.cpp:33-64. PLANE (Parameters::is3D false): sigma_zz,
+// _yz, _zx keep their values (ConstitutiveModels.cpp:59-63).
+template <bool PLANE = false>
 __device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6], const float e[6],
                                              float s[6], float dt) {
   const float vol = 0.33333f * (er[0] + er[1] + er[2]);
@@ -35,8 +37,9 @@ __device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6],
   const float c = M.visc * (2.0f - (2.0f / 3.0f));
   s[0] = P + c * (er[0] - vol) * dt;
   s[1] = P + c * (er[1] - vol) * dt;
-  s[2] = P + c * (er[2] - vol) * dt;
+  if (!PLANE) s[2] = P + c * (er[2] - vol) * dt;
   s[3] = 2.0f * M.visc * er[3] * dt;
+  if (PLANE) return;
   s[4] = 2.0f * M.visc * er[4] * dt;
   s[5] = 2.0f * M.visc * er[5] * dt;
 }
@@ -46,6 +49,8 @@ __device__ __forceinline__ void cm_newtonian(const MatDev &M, const float er[6],
 // the turbulence flag: a set flag halves the viscosity of this update, then the flag is
 // set when KE - VE - turbStress > 0 and cleared (with the viscosity restored) otherwise.
 // KE = m |v|^2 / 2 with the velocity after this step's update, VE = mu (shear rates^2) / 2.
+// PLANE: sigma_zz, _yz, _zx keep their values (ConstitutiveModels.cpp:121-125).
+template <bool PLANE = false>
 __device__ __forceinline__ void cm_turbulent(const MatDev &M, const float er[6], const float e[6],
                                              const float v[3], float mass, float &dmg, float s[6], float dt) {
   float mu = M.visc;
@@ -62,13 +67,17 @@ __device__ __forceinline__ void cm_turbulent(const MatDev &M, const float er[6],
   const float c = mu * (2.0f - (2.0f / 3.0f));
   s[0] = P + c * er[0] * dt;
   s[1] = P + c * er[1] * dt;
-  s[2] = P + c * er[2] * dt;
   s[3] = 2.0f * mu * er[3] * dt;
+  if (PLANE) return;
+  s[2] = P + c * er[2] * dt;
   s[4] = 2.0f * mu * er[4] * dt;
   s[5] = 2.0f * mu * er[5] * dt;
 }
 
-// perfectPlasticity, ConstitutiveModels.cpp:131-234
+// perfectPlasticity, ConstitutiveModels.cpp:131-234. PLANE: the trial sigma_zz, _yz, _zx
+// are zero (ConstitutiveModels.cpp:154-160), so the elastic branch writes zeros there and
+// the plastic branch leaves them (ConstitutiveModels.cpp:217-222).
+template <bool PLANE = false>
 __device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], float s[6], float p[6],
                                            float &epb, float dt) {
   const float G = M.G, K = M.K;
@@ -76,10 +85,10 @@ __device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], f
   float tr[6];
   tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
   tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
-  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
+  tr[2] = PLANE ? 0.0f : s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
   tr[3] = s[3] + (2.0f * G * er[3]) * dt;
-  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
-  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
+  tr[4] = PLANE ? 0.0f : s[4] + (2.0f * G * er[4]) * dt;
+  tr[5] = PLANE ? 0.0f : s[5] + (2.0f * G * er[5]) * dt;
   const float a01 = tr[0] - tr[1], a12 = tr[1] - tr[2], a20 = tr[2] - tr[0];
   const float q = sqrtf(0.5f * a01 * a01 + 0.5f * a12 * a12 + 0.5f * a20 * a20 + tr[3] * tr[3] +
                         tr[4] * tr[4] + tr[5] * tr[5]);
@@ -107,8 +116,9 @@ __device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], f
   }
   s[0] += (2.0f * G * (el[0] - vol) + 3.0f * K * vol) * dt;
   s[1] += (2.0f * G * (el[1] - vol) + 3.0f * K * vol) * dt;
-  s[2] += (2.0f * G * (el[2] - vol) + 3.0f * K * vol) * dt;
   s[3] += (2.0f * G * el[3]) * dt;
+  if (PLANE) return;
+  s[2] += (2.0f * G * (el[2] - vol) + 3.0f * K * vol) * dt;
   s[4] += (2.0f * G * el[4]) * dt;
   s[5] += (2.0f * G * el[5]) * dt;
 }
@@ -264,6 +274,45 @@ __device__ void gk_damage(const MatDev &M, float s[6], float &D, float &smax, fl
     o[3] += x * l * y; o[4] += y * l * z; o[5] += x * l * z;
   }
   for (int k = 0; k < 6; k++) s[k] = (float)o[k];
+}
+
+// gradyKippDamage 2D branch (DamageModels.cpp:11-69), in FP64: principal stresses of the
+// in-plane tensor by the rotation angle theta (pi/4 when |sxx - syy| < 1e-5, with
+// pi = 3.14159 as there), damage growth of the tensile ones, and sigma_zz = 0 whenever the
+// stress is rebuilt (growth or complete damage); otherwise the stress is left as read.
+__device__ __noinline__ void gk_damage_plane(const MatDev &M, float s[6], float &D, float &smax, float mass, float rho,
+                                float dt) {
+  const double sxx = s[0], syy = s[1], sxy = s[3];
+  const double th = (fabs(sxx - syy) < 1e-5) ? 3.14159 / 4.0 : 0.5 * atan(2.0 * sxy / (sxx - syy));
+  const double c = cos(th), sn = sin(th);
+  double s1 = sxx * c * c + syy * sn * sn + 2.0 * c * sn * sxy;
+  double s2 = sxx * sn * sn + syy * c * c - 2.0 * c * sn * sxy;
+  smax = (float)fmax(s1, fmax(s2, 0.0));
+  const double smx = fmax(s1, fmax(s2, 0.0));
+  double Dn = D;
+  bool rebuild = false;
+  if (smx > 1e-3) {
+    const double eff = smx / M.gk_K43;
+    const double vol = (double)mass / (double)rho;
+    const double emin = pow(vol * M.gk_k, -1.0 / M.gk_m);
+    if (eff > emin) {
+      const double rate = M.gk_m3 * M.gk_alpha13 * pow(eff, M.gk_m / 3.0);
+      Dn = pow(pow(Dn, 1.0 / 3.0) + (double)dt * rate, 3.0);
+      if (Dn > 1.0) Dn = 1.0;
+      rebuild = true;
+    }
+  }
+  if (Dn == 1.0) rebuild = true;
+  D = (float)Dn;
+  if (!rebuild) return;
+  // the reference rebuilds once more when D is 1 after the growth; that second pass scales
+  // principal stresses already scaled to zero and changes nothing
+  if (s1 > 0.0) s1 = (1.0 - Dn) * s1;
+  if (s2 > 0.0) s2 = (1.0 - Dn) * s2;
+  s[0] = (float)(s1 * c * c + s2 * sn * sn);
+  s[1] = (float)(s1 * sn * sn + s2 * c * c);
+  s[3] = (float)((s1 - s2) * c * sn);
+  s[2] = 0.0f;
 }
 
 }  // namespace

@@ -347,18 +347,20 @@ def brazilian_disk_2d(h=0.002, L=0.2, R=0.04, v=0.2, steps=300) -> Scene:
     return s
 
 
-def square_2d(h=0.02, L=0.5, side=0.2, vx=3.0, steps=60, mat=None) -> Scene:
+def square_2d(h=0.02, L=0.5, side=0.2, vx=3.0, steps=60, mat=None, centered=False, squeeze=0.0) -> Scene:
     """2D periodic plane box with a square of side `side` near the +x face moving at vx
     with a shear v_y = 0.5 (x - x_c)/side, so it crosses the periodic wrap at L - 2h within
     tens of steps. Elastic (K 1e6, G 5e5, rho 1000) unless `mat` is given; gravity -9.81
-    along y; dt = 0.2 h / c_p."""
+    along y; dt = 0.2 h / c_p. `centered` puts the square in the middle of the box;
+    `squeeze` adds a converging v_x = -2 squeeze (x - x_c)/side (squeeze m/s at the faces)."""
     psep = h / 2
     n = int(round(side / psep))
-    lo = (L - 2 * h - side, (L - side) / 2)
+    lo = ((L - side) / 2 if centered else L - 2 * h - side, (L - side) / 2)
     pos = plane_lattice(lo, (n, n), psep)
     vel = np.zeros_like(pos)
-    vel[:, 0] = vx
-    vel[:, 1] = 0.5 * (pos[:, 0] - (lo[0] + side / 2)) / side
+    xc = lo[0] + side / 2
+    vel[:, 0] = vx - 2.0 * squeeze * (pos[:, 0] - xc) / side
+    vel[:, 1] = 0.5 * (pos[:, 0] - xc) / side
     K, G, rho = 1e6, 5e5, 1000.0
     m = material(model=1, bulk_mod=K, shear_mod=G, density=rho) if mat is None else mat
     s = Scene("square_2d", (L, L, 0.0), h, psep, [m], gravity=(0.0, -9.81, 0.0),

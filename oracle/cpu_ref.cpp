@@ -69,6 +69,9 @@ struct Sim {
   // ---- parameters ----
   double g[3] = {0, 0, 0};
   double damping = 0, mu = 0, dt = 0, tlast = 0;
+  // Sort cadence: the reference sorts at steps that are multiples of 10 and at step 1
+  // (Solver.cpp:91-94). 1 models a re-binning every step (test knob, not the reference).
+  int sort_every = 10;
   std::vector<gy_material> mats;
 };
 
@@ -868,7 +871,7 @@ void update_particle(Sim &s, int mp) {
 // One step — Solver::iterate_CPU, Solver.cpp:86-292. Grid BC phases 1 and 3 write the
 // same values as phase 2 (Grid.cpp:862-1094 ignores `phase`), so only phase 2 is run.
 void iterate(Sim &s, double t, int step) {
-  if ((step % 10 == 0) || (step == 1)) sort_by_cell(s);
+  if ((step % s.sort_every == 0) || (step == 1)) sort_by_cell(s);
   const int N = s.numNodes, P = s.np;
   // The reference's chunk N/(omp_get_num_threads()*16) (Solver.cpp:120-249) is evaluated
   // inside the team, so it is 0 (libgomp then never advances a dynamic loop: the step
@@ -1027,6 +1030,7 @@ int32_t cref_set_node_bc(void *h, int32_t n, const int32_t *type, const double *
 }
 
 double cref_dt(void *h) { return ((Sim *)h)->dt; }
+void cref_set_sort_every(void *h, int32_t k) { ((Sim *)h)->sort_every = k > 0 ? k : 10; }
 void cref_set_dt(void *h, double dt) { ((Sim *)h)->dt = dt; }
 int32_t cref_num_nodes(void *h) { return ((Sim *)h)->numNodes; }
 

@@ -83,7 +83,9 @@ OUT_FIELDS = ["pos", "vel", "stress", "strain", "plastic_strain", "acc_plastic_s
 class CpuSim:
     """One oracle simulation of a `graphyt_amd.workloads.Scene`."""
 
-    def __init__(self, scene, kind: str = "restatement"):
+    def __init__(self, scene, kind: str = "restatement", sort_every: int | None = None):
+        """sort_every (restatement only): re-bin every k steps instead of the reference's
+        cadence (steps divisible by 10, and step 1); 1 re-bins every step."""
         self.kind = kind
         if kind == "reference":
             # The reference's dynamic OpenMP chunk, P/(16*threads), is evaluated inside the
@@ -109,6 +111,11 @@ class CpuSim:
             rc = self._f("set_node_bc")(self.h, len(t), t.ctypes.data_as(C.POINTER(C.c_int32)), _dp(v))
             if rc != 0:
                 raise RuntimeError("oracle set_node_bc failed")
+        if sort_every is not None:
+            if kind != "restatement":
+                raise ValueError("sort_every is a knob of the restatement only")
+            self.lib.cref_set_sort_every.argtypes = [C.c_void_p, C.c_int32]
+            self.lib.cref_set_sort_every(self.h, int(sort_every))
 
     def _f(self, name):
         return getattr(self.lib, self.pre + name)

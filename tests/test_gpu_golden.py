@@ -2,6 +2,11 @@
 own compiled code (tests/golden/make_golden.py). Tolerances are relative L2 norms over all
 particles, ||gpu - reference|| / ||reference||, per scene: FP32 state against the FP64
 reference accumulates rounding per step, and plastic and damage branches widen it.
+
+Scenes whose particles cross the periodic wrap (scenes.WRAPPING) are compared with the
+oracle re-binning every step, as the GPU path does: there the reference's sort cadence
+drops a wrapped particle's P2G contributions until its next sort (test_golden.py pins the
+oracle to the fixture at the reference's cadence and shows the difference).
 """
 import os
 import sys
@@ -28,14 +33,21 @@ TOL = {
     "bcs": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "rigid_platen": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "two_blocks_mu": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
+    "plane_dam_break": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "strain": 1e-4, "density": 1e-6},
+    "plane_elastic": ELASTIC,
+    "plane_perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
+    "plane_turbulent": ELASTIC,
+    "plane_grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
 }
+# plane scenes the GPU path runs (the plane Brazilian disk needs contact between objects)
+PLANE_GPU = [n for n in scenes.PLANE_NAMES if n != "plane_brazilian"]
 
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def gpu_run(name):
+def gpu_run(name, oracle_mod=None):
     sc, steps, t_of = scenes.build(name)
     g = runtime.Context(sc)
     if t_of is None:
@@ -48,16 +60,22 @@ def gpu_run(name):
     with np.load(os.path.join(HERE, "golden", f"{name}.npz")) as f:
         fx = {k: f[k] for k in f.files}
     assert np.allclose(scenes.input_digest(sc), fx["digest"], rtol=1e-12, atol=0), "scene builder changed"
+    if name in scenes.WRAPPING:
+        sim = oracle_mod.CpuSim(sc, "restatement", sort_every=1)
+        for s in range(steps):
+            sim.iterate(s * sim.dt if t_of is None else t_of(s), s)
+        fx = sim.get(scenes.fields(name))
+        sim.close()
     return out, fx
 
 
-@pytest.mark.parametrize("name", [n for n in scenes.NAMES if n != "dp_unguarded"])
-def test_gpu_matches_reference_fixture(name):
-    out, fx = gpu_run(name)
+@pytest.mark.parametrize("name", [n for n in scenes.NAMES if n != "dp_unguarded"] + PLANE_GPU)
+def test_gpu_matches_reference_fixture(name, oracle_mod):
+    out, fx = gpu_run(name, oracle_mod)
     for k, tol in TOL[name].items():
         r = rel(out[k], fx[k])
         assert r <= tol, f"{name} {k}: rel L2 {r:.3e} > {tol:.1e}"
-    if name == "turbulent":
+    if name in ("turbulent", "plane_turbulent"):
         # the flag follows KE - VE - turbStress > 0, evaluated in FP32 here and FP64 there:
         # only particles within rounding of the threshold may differ
         assert np.sum(out["damage"] != fx["damage"]) <= max(2, out["damage"].size // 100)
@@ -79,3 +97,18 @@ def test_gpu_unguarded_apex_nan_share():
     assert nb.sum() > 0 and 0.5 * nb.sum() < na.sum() < 2.0 * nb.sum()
     both = ~na & ~nb
     assert rel(out["pos"][both], fx["pos"][both]) <= 1e-6
+
+
+def test_gpu_plane_z_is_not_advanced():
+    """Plane path: z positions and z velocities stay as uploaded (TimeIntegration.cpp:30-40),
+    and the stencil never leaves the single node layer."""
+    out, fx = gpu_run("plane_dam_break")
+    sc, _, _ = scenes.build("plane_dam_break")
+    assert np.array_equal(out["pos"][:, 2], sc.particles["pos"][:, 2])
+    assert np.all(out["vel"][:, 2] == 0.0)
+
+
+def test_gpu_plane_contact_is_rejected():
+    sc, _, _ = scenes.build("plane_brazilian")
+    with pytest.raises(runtime.GyError, match="plane"):
+        runtime.Context(sc)
