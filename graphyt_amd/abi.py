@@ -67,7 +67,7 @@ class GyParticles(C.Structure):
         ("plastic_strain", _PD), ("acc_plastic_strain", _PD), ("density", _PD), ("mass", _PD),
         ("damage", _PD), ("smax", _PD), ("material", _PI), ("obj_id", _PI), ("hold", _PU8),
         ("bc_type", _PI), ("bc_val", _PD), ("volume", _PD), ("displacement", _PD),
-        ("cell_id", _PI),
+        ("cell_id", _PI), ("temperature", _PD), ("temperature_grad", _PD),
     ]
 
 
@@ -87,7 +87,7 @@ PARTICLE_FIELDS = {
     "damage": (1, np.float64), "smax": (1, np.float64), "material": (1, np.int32),
     "obj_id": (1, np.int32), "hold": (1, np.uint8), "bc_type": (1, np.int32),
     "bc_val": (4, np.float64), "volume": (1, np.float64), "displacement": (3, np.float64),
-    "cell_id": (1, np.int32),
+    "cell_id": (1, np.int32), "temperature": (1, np.float64), "temperature_grad": (3, np.float64),
 }
 _CTYPE = {np.float64: _PD, np.int32: _PI, np.uint8: _PU8}
 

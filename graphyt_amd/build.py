@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libgraphyt_amd.so")
-SOURCES = ["gy_kernels.hip", "gy_plane.hip", "gy_ctx.hip", "gy_mlp.hip"]
+SOURCES = ["gy_kernels.hip", "gy_plane.hip", "gy_thermal.hip", "gy_ctx.hip", "gy_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GY_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -12,10 +12,11 @@ step)` / `Solver.restore(path)`, written in C++) and the C-ABI contexts of
 
 Particle arrays use the names of `abi.PARTICLE_FIELDS` (pos, pos0, vel, stress, strain,
 plastic_strain, acc_plastic_strain, density, mass, volume, damage, smax, material,
-obj_id, hold, bc_type, bc_val, cell_id, ...), node BCs are node_bc_type / node_bc_val,
-and the grid is grid_L, grid_h, psep. FP32 device values and fixed-point positions
-survive the FP64 round trip exactly, so a restored run continues from the same state; it
-is re-binned on upload, which reorders the FP32 sums only (agreement to rounding).
+obj_id, hold, bc_type, bc_val, cell_id, temperature, temperature_grad), node BCs are
+node_bc_type / node_bc_val, and the grid is grid_L, grid_h, psep. FP32 device values and
+fixed-point positions survive the FP64 round trip exactly, so a restored run continues
+from the same state; it is re-binned on upload, which reorders the FP32 sums only
+(agreement to rounding).
 """
 from __future__ import annotations
 
@@ -30,7 +31,7 @@ _CODES = {np.dtype(np.float64): 0, np.dtype(np.int32): 1, np.dtype(np.uint8): 2}
 
 PARTICLE_NAMES = ["pos", "pos0", "vel", "stress", "strain", "plastic_strain", "acc_plastic_strain",
                   "density", "mass", "volume", "damage", "smax", "material", "obj_id", "hold",
-                  "bc_type", "bc_val", "cell_id"]
+                  "bc_type", "bc_val", "cell_id", "temperature", "temperature_grad"]
 
 
 def write(path: str, arrays: dict, t: float, step: int, dt: float = 0.0) -> None:
@@ -103,6 +104,8 @@ def load(path: str, scene):
         raise ValueError("checkpoint grid differs from the scene's")
     pa = ParticleArrays(n)
     for k in PARTICLE_NAMES:
+        if k not in arrays:  # e.g. temperature, absent from files written before it was stored
+            continue
         comp, dtype = PARTICLE_FIELDS[k]
         a = arrays[k].astype(dtype)
         pa.arrays[k] = a.reshape(n, comp) if comp > 1 else a

@@ -20,7 +20,7 @@ namespace {
 
 const char *kKernelNames[K_COUNT_] = {"keys",  "scan_partial", "scan_final", "place",   "p2g",
                                       "grid",  "g2p2g",        "advance",    "audit",   "order",
-                                      "mb_claim", "mb_p2g"};
+                                      "mb_claim", "mb_p2g", "therm_p2g", "therm_g2p"};
 
 struct Timed {
   int id;
@@ -154,6 +154,8 @@ MatDev make_mat(const gy_material &m) {
   M.carry_plastic = (m.model == 3 || m.model == 4);
   M.carry_damage = M.gk || m.model == 5;  // model 5 keeps its turbulence flag in the damage field
   M.turb = (float)m.turb_stress;
+  M.cp = (float)m.specific_heat;
+  M.kt = (float)m.thermal_conductivity;
   M.K = (float)m.bulk_mod;
   M.G = (float)m.shear_mod;
   M.visc = (float)m.viscosity;
@@ -276,6 +278,7 @@ int alloc_grid(gy_ctx *c) {
   }
   c->nnodes = node_base;
   DevPtrs &d = c->d;
+  d.nnodes = node_base;
   d.ninst = (int)c->inst.size();
   d.ntiles = tile_base;
   d.nbricks = brick_base;
@@ -335,16 +338,20 @@ int check_status(gy_ctx *c) {
 // One step after the binning of the current buffer is in place.
 void enqueue_step(gy_ctx *c) {
   void *s = c->stream;
+  const bool th = c->d.thermal != 0;
   if (c->plane) {
     timed(c, K_GRID, [&] { launch_plane_grid(c->d, s); });
+    if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
     timed(c, K_G2P2G, [&] { launch_plane_g2p2g(c->d, s); });
     timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
     timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
     timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;
   }
   if (c->d.multibody) {
     timed(c, K_GRID, [&] { launch_mb_grid(c->d, s); });
+    if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
     timed(c, K_G2P2G, [&] { launch_mb_update(c->d, s); });
     timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
     timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
@@ -352,13 +359,16 @@ void enqueue_step(gy_ctx *c) {
     timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
     timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
+    if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;
   }
   timed(c, K_GRID, [&] { launch_grid(c->d, s); });
+  if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
   timed(c, K_G2P2G, [&] { launch_g2p2g(c->d, s); });
   timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });  // with the step bookkeeping
   timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
   timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+  if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
 }
 
 // First step after an upload: bin the uploaded state and scatter it to the grid.
@@ -371,6 +381,7 @@ void enqueue_init(gy_ctx *c) {
     timed(c, K_PLACE, [&] { launch_place(c->d, s); });
     timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
     launch_brick_list(c->d, s);
+    if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;
   }
   timed(c, K_KEYS, [&] { launch_keys(c->d, s); });
@@ -381,10 +392,12 @@ void enqueue_init(gy_ctx *c) {
     timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
     timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
+    if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;
   }
   timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
   launch_brick_list(c->d, s);  // within a step the scan builds it
+  if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
 }
 
 void drop_graph(gy_ctx *c) {
@@ -496,7 +509,7 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
   if (ninst != (int)c->inst.size())
     return fail(c, GY_ERR_INVALID, "upload of %d instance(s) into a context of %d", ninst, (int)c->inst.size());
   long long total = 0;
-  bool multibody = false;
+  bool multibody = false, thermal = false;
   for (int k = 0; k < ninst; k++) {
     const gy_particles *in = &views[k];
     if (!in->pos || in->n <= 0) return fail(c, GY_ERR_INVALID, "instance %d: upload needs n > 0 and positions", k);
@@ -513,6 +526,7 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
       if (in->obj_id && in->obj_id[i] != obj0) multibody = true;
       if (in->bc_type && (in->bc_type[i] < 0 || in->bc_type[i] > 127))
         return fail(c, GY_ERR_INVALID, "instance %d particle %d: bad BC type", k, i);
+      if (in->temperature && in->temperature[i] != 0.0) thermal = true;
     }
     total += in->n;
   }
@@ -580,6 +594,9 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
       put(F_RHO, s_, rho);
       put(F_MASS, s_, mass);
       if (in->damage) { put(F_DMG, s_, in->damage[i]); if (in->damage[i] != 0.0) H.carry_damage[m] = 1; }
+      if (in->temperature) put(F_T, s_, in->temperature[i]);
+      if (in->temperature_grad)
+        for (int a = 0; a < 3; a++) put(F_TG0 + a, s_, in->temperature_grad[3 * i + a]);
       if (in->smax) { put(F_SMAX, s_, in->smax[i]); if (in->smax[i] != 0.0) H.carry_damage[m] = 1; }
       const int bct = in->bc_type ? in->bc_type[i] : 0;
       F(F_TAG, s_) = make_tag(m, in->hold ? (in->hold[i] ? 1 : 0) : 0, bct, in->obj_id ? in->obj_id[i] : 0);
@@ -617,6 +634,8 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     return fail(c, GY_ERR_UNSUPPORTED, "plane (is3D = false) contexts hold one deformable object (no contact yet)");
   d.kmask = kmask;
   d.multibody = multibody ? 1 : 0;
+  d.thermal = thermal ? 1 : 0;
+  if (thermal && (rc = dalloc(c, &d.therm, 2 * (size_t)c->nnodes))) return rc;
   if (multibody) {
     if ((rc = dalloc(c, &d.acc2, (size_t)c->nnodes * MB_ACC2_W))) return rc;
     if ((rc = dalloc(c, &d.nrm, (size_t)c->nnodes * 6))) return rc;
@@ -749,6 +768,9 @@ static int download_impl(gy_ctx *c, int inst, gy_particles *o) {
     if (o->volume) o->volume[i] = mass / rho;
     if (o->damage) o->damage[i] = cd ? Fv(F_DMG, s) : 0.0;
     if (o->smax) o->smax[i] = cd ? Fv(F_SMAX, s) : 0.0;
+    if (o->temperature) o->temperature[i] = c->d.thermal ? Fv(F_T, s) : 0.0;
+    if (o->temperature_grad)
+      for (int a = 0; a < 3; a++) o->temperature_grad[3 * i + a] = c->d.thermal ? Fv(F_TG0 + a, s) : 0.0;
     if (o->material) o->material[i] = m;
     if (o->obj_id) o->obj_id[i] = tag_obj(tag);
     if (o->hold) o->hold[i] = (uint8_t)tag_hold(tag);

@@ -62,6 +62,8 @@ enum PField : int {
   F_SMAX,                                   // max principal stress
   F_TAG,                                    // packed: mat(8) | hold(1) | bc type(7) | obj(16)
   F_PID,                                    // caller's particle index
+  F_T,                                      // temperature
+  F_TG0, F_TG1, F_TG2,                      // temperature gradient of the last G2P
   NF
 };
 
@@ -115,6 +117,7 @@ struct MatDev {
   int carry_damage;   // damage / smax fields are live for this material (Grady-Kipp, model 5's flag)
   float K, G, visc, sy;
   float turb;         // turbStress of model 5
+  float cp, kt;       // specificHeatCapacity, thermalConductivity
   float c, ecr, eta, xi, eta_bar;
   float c_over_ecr, c_floor;   // c/eps_cr, c/10
   float xi_over_eta, xi_over_etab;
@@ -188,6 +191,10 @@ struct DevPtrs {
   int4 *claim;           // per node: object of body 1, object of body 2, rigid flags
   float *velb;           // MB_VELB_W per node: v1, a1, v2, a2 after the contact solve
   int4 *cellobj;         // per key: first and second object in the cell, rigid flags
+  // heat transfer (any uploaded temperature nonzero); see gy_thermal.hip
+  int thermal;
+  float *therm;          // 2 per node: heat content sum(N m cp T), conduction rate
+  long long nnodes;      // nodes of all instances
 };
 
 constexpr int MB_ACC2_W = 16;   // per canonical node: [m p3 f3 mr] for body 1, then body 2
@@ -196,7 +203,7 @@ constexpr int MB_VELB_W = 12;   // per node: v1 a1 v2 a2
 // Kernel identifiers for per-kernel timing.
 enum KernelId : int {
   K_KEYS = 0, K_SCAN_PARTIAL, K_SCAN_FINAL, K_PLACE, K_P2G, K_GRID, K_G2P2G, K_ADVANCE,
-  K_AUDIT, K_ORDER, K_MB_CLAIM, K_MB_P2G, K_COUNT_
+  K_AUDIT, K_ORDER, K_MB_CLAIM, K_MB_P2G, K_THERM_P2G, K_THERM_G2P, K_COUNT_
 };
 
 // Device material constants of a gy_material (defined in gy_ctx.hip).
@@ -231,5 +238,9 @@ void launch_plane_keys(const DevPtrs &d, void *stream);
 void launch_plane_p2g_only(const DevPtrs &d, void *stream);
 void launch_plane_g2p2g(const DevPtrs &d, void *stream);
 void launch_plane_grid(const DevPtrs &d, void *stream);
+// heat transfer (gy_thermal.hip): node sums of the current state (cleared first), and the
+// particle gradient, rate and temperature update before the fused particle kernel
+void launch_therm_p2g(const DevPtrs &d, void *stream);
+void launch_therm_g2p(const DevPtrs &d, void *stream);
 
 }  // namespace gy

@@ -664,6 +664,10 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     B[F_MASS * cap + out] = u32(mass);
     B[F_TAG * cap + out] = tag;
     B[F_PID * cap + out] = (uint32_t)pid;
+    if (d.thermal) {  // temperature and gradient travel with the particle (gy_thermal.hip)
+#pragma unroll
+      for (int a = 0; a < 4; a++) B[(F_T + a) * cap + out] = A[(F_T + a) * cap + slot];
+    }
 
     // ---- new binning key ----
     bad = false;

@@ -307,7 +307,8 @@ struct MaterialPoints {
     v.acc_plastic_strain = acc_plastic_strain.data(); v.density = density.data(); v.mass = mass.data();
     v.damage = damage.data(); v.smax = smax.data(); v.material = material.data(); v.obj_id = objID.data();
     v.hold = hold.data(); v.bc_type = BCTYPE.data(); v.bc_val = BCVAL.data(); v.volume = volume.data();
-    v.displacement = displacement.data(); v.cell_id = cellID.data();
+    v.displacement = displacement.data(); v.cell_id = cellID.data(); v.temperature = temperature.data();
+    v.temperature_grad = temperatureGrad.data();
     return v;
   }
 
@@ -316,6 +317,7 @@ struct MaterialPoints {
   double psep = 0.0;
   std::vector<double> position, pos0, displacement, velocity, acceleration, stress, strain, plastic_strain,
       acc_plastic_strain, damage, smax, density, volume, mass, temperature, normals, BCVAL;
+  std::vector<double> temperatureGrad;  // MaterialPoints::temperatureGrad (3 per particle)
   std::vector<int32_t> cellID, material, objID, BCTYPE;
   std::vector<uint8_t> hold;
   bool host_dirty = true;    // host arrays changed since the last upload
@@ -332,6 +334,7 @@ void MaterialPoints::add(int i, double x, double y, double z) {
     velocity.push_back(0.0);
     acceleration.push_back(0.0);
     normals.push_back(0.0);
+    temperatureGrad.push_back(0.0);
   }
   for (int k = 0; k < 6; k++) {
     stress.push_back(0.0);
@@ -386,15 +389,17 @@ class Solver {
     if (ctx) gy_destroy(ctx);
   }
 
-  // Solver::initialize (Solver.cpp:39-56), 3D branch
+  // Solver::initialize (Solver.cpp:39-67): per unit depth in the plane path, where rigid
+  // particles keep rho * V
   void initialize() {
-    const double v = mpts->psep * mpts->psep * mpts->psep;
+    const bool d3 = prm->is3D;
+    const double v = d3 ? mpts->psep * mpts->psep * mpts->psep : mpts->psep * mpts->psep;
     for (size_t p = 0; p < mpts->size(); p++) {
       const int m = mpts->material[p];
       if (m < 0 || (size_t)m >= materials.size()) throw std::invalid_argument("particle material out of range");
       mpts->density[p] = materials[m].density;
       mpts->volume[p] = v;
-      mpts->mass[p] = materials[m].isRigid ? 1.0 : mpts->density[p] * v;
+      mpts->mass[p] = (d3 && materials[m].isRigid) ? 1.0 : mpts->density[p] * v;
     }
     mpts->host_dirty = true;
   }
@@ -458,7 +463,9 @@ class Solver {
         {"acc_plastic_strain", 0, P.acc_plastic_strain.size(), P.acc_plastic_strain.data()},
         {"density", 0, P.density.size(), P.density.data()}, {"mass", 0, P.mass.size(), P.mass.data()},
         {"volume", 0, P.volume.size(), P.volume.data()}, {"damage", 0, P.damage.size(), P.damage.data()},
-        {"smax", 0, P.smax.size(), P.smax.data()}, {"material", 1, P.material.size(), P.material.data()},
+        {"smax", 0, P.smax.size(), P.smax.data()}, {"temperature", 0, P.temperature.size(), P.temperature.data()},
+        {"temperature_grad", 0, P.temperatureGrad.size(), P.temperatureGrad.data()},
+        {"material", 1, P.material.size(), P.material.data()},
         {"obj_id", 1, P.objID.size(), P.objID.data()}, {"hold", 2, P.hold.size(), P.hold.data()},
         {"bc_type", 1, P.BCTYPE.size(), P.BCTYPE.data()}, {"bc_val", 0, P.BCVAL.size(), P.BCVAL.data()},
         {"cell_id", 1, P.cellID.size(), P.cellID.data()},
@@ -548,6 +555,8 @@ class Solver {
       else if (name == "volume") fill(P.volume, 0, n);
       else if (name == "damage") fill(P.damage, 0, n);
       else if (name == "smax") fill(P.smax, 0, n);
+      else if (name == "temperature") fill(P.temperature, 0, n);
+      else if (name == "temperature_grad") fill(P.temperatureGrad, 0, 3 * n);
       else if (name == "material") fill(P.material, 1, n);
       else if (name == "obj_id") fill(P.objID, 1, n);
       else if (name == "hold") fill(P.hold, 2, n);
@@ -580,8 +589,10 @@ class Solver {
 
  private:
   void prepare(Parameters *p) {
-    if (p && p != prm) prm = p;
-    if (!prm->is3D) throw std::runtime_error("iterate_GPU: only the 3D path is built (Parameters is3D = True)");
+    if (p && p != prm) {
+      if (ctx && p->is3D != prm->is3D) throw std::runtime_error("iterate_GPU: is3D changed after the first step");
+      prm = p;
+    }
     if (!ctx) {
       gy_grid g{};
       for (int a = 0; a < 3; a++) g.L[a] = nds->L[a];
@@ -595,7 +606,7 @@ class Solver {
       gp.cfl = prm->CFL;
       gp.psep = mpts->psep;
       gp.dt = dt;
-      gp.is3D = 1;
+      gp.is3D = prm->is3D ? 1 : 0;
       gy_ctx *c = nullptr;
       const int rc = gy_create(&c, &g, mats.data(), (int)mats.size(), &gp);
       if (rc != GY_OK) {

@@ -129,6 +129,11 @@ typedef struct gy_particles {
   double  *volume;             /* n  : mass / density                       */
   double  *displacement;       /* 3n : pos - pos0                           */
   int32_t *cell_id;            /* n  : reference cellID (TimeIntegration.cpp:110-115) */
+  /* appended after the derived fields (layout of the fields above unchanged) */
+  double  *temperature;        /* n  : MaterialPoints::temperature (setTemp); heat
+                                *      transfer runs when any uploaded value is nonzero */
+  double  *temperature_grad;   /* 3n : MaterialPoints::temperatureGrad of the last G2P
+                                *      (0 after MaterialPoints::add); kept for restarts */
 } gy_particles;
 
 /* FP64 audit of the device state (sums over all particles of every instance). */

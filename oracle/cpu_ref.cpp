@@ -12,8 +12,9 @@
 //
 // Scope: 3D and 2D (Parameters::is3D, the reference's per-unit-depth plane path);
 // constitutive models 1-5; Grady-Kipp damage (both branches); one- and two-body contact
-// with rigid bodies; grid and particle BCs; periodic wrap. Thermal fields are not
-// simulated (T stays 0, as in every configuration studied here).
+// with rigid bodies; grid and particle BCs; periodic wrap; the thermal fields
+// (temperature, its gradient and rate: Extrapolation.cpp:281-283, 359-362,
+// TimeIntegration.cpp:50).
 //
 // Parity notes:
 //  * Grady-Kipp uses a cyclic Jacobi eigensolver (jacobi_eig3 below) instead of the
@@ -55,12 +56,18 @@ struct Sim {
   std::vector<int> nbcType;                // per node
   std::vector<double> nbcVal;              // 4 per node
   std::vector<int> cellCount, cellStart;   // per cell (sized numNodes as in Nodes.cpp:36-37)
+  std::vector<double> nT, nTrate, nTmass;  // thermal: heat content, its rate, thermal mass
 
   // ---- particles (MaterialPoints.hpp:84-137), interleaved per particle ----
   int np = 0;
   double psep = 0;
   std::vector<double> x, x0, u, vel, gvel, acc, erate, sig, eps, epsp, epbar, dmg, smax,
-      mass, rho, vol, nrm, drate;
+      mass, rho, vol, nrm, drate, T, Tgrad, Trate;
+  // the reference's sort scratch for the temperature gradient, which persists between sorts
+  std::vector<double> Tgrad_tmp;
+  // false: the gradient moves with its particle at a sort (a test knob modelling the GPU
+  // path); true: the reference's permutation (see sort_by_cell)
+  bool tgrad_as_reference = true;
   std::vector<int> cell, mat, obj, pbcType, pid;
   std::vector<char> hold;
   std::vector<double> pbcVal;  // 4 per particle
@@ -156,6 +163,19 @@ void sort_by_cell(Sim &s) {
   permute(s.rho, o, 1); permute(s.vol, o, 1); permute(s.nrm, o, 3); permute(s.cell, o, 1);
   permute(s.mat, o, 1); permute(s.obj, o, 1); permute(s.hold, o, 1); permute(s.pbcVal, o, 4);
   permute(s.pbcType, o, 1); permute(s.pid, o, 1);
+  permute(s.T, o, 1);
+  // The reference gathers the gradient (3 per particle) with a scalar index, into the first
+  // np entries of its scratch array, and swaps the arrays (Extrapolation.cpp:446, 485): the
+  // gradient after a sort holds component (j mod 3) of particle j/3, j = pidx[i], in its
+  // first np entries and the scratch's previous contents in the rest. The rate is swapped
+  // with a scratch array that is never written (Extrapolation.cpp:486); it is reset before
+  // every use (TimeIntegration.cpp:160), so it is left alone here.
+  if (s.tgrad_as_reference) {
+    for (int i = 0; i < s.np; i++) s.Tgrad_tmp[i] = s.Tgrad[o[i]];
+    s.Tgrad.swap(s.Tgrad_tmp);
+  } else {
+    permute(s.Tgrad, o, 3);
+  }
   int run = 0;
   s.cellStart[0] = 0;
   for (int c = 1; c < ncell; c++) {
@@ -165,8 +185,9 @@ void sort_by_cell(Sim &s) {
 }
 
 // ------------------------------------------------------------------------------------
-// resetNode — Grid.cpp:797-860 (thermal fields omitted).
+// resetNode — Grid.cpp:797-860.
 void reset_node(Sim &s, int n) {
+  s.nT[n] = s.nTrate[n] = s.nTmass[n] = 0.0;
   for (int k = 0; k < 3; k++) {
     s.v[3 * n + k] = s.vb1[3 * n + k] = s.vb2[3 * n + k] = 0.0;
     s.a[3 * n + k] = s.ab1[3 * n + k] = s.ab2[3 * n + k] = 0.0;
@@ -181,7 +202,7 @@ void reset_node(Sim &s, int n) {
 
 // ------------------------------------------------------------------------------------
 // One particle -> one node contribution — extrapolate_Matpnt_to_Node,
-// Extrapolation.cpp:166-286 (3D, thermal terms omitted). Operation order is kept.
+// Extrapolation.cpp:166-286. Operation order is kept.
 void p2g_pair(Sim &s, int mp, int n) {
   const Shape w = shape(s, mp, n);
   const double sx = w.sx, sy = w.sy, sz = w.sz, gx = w.gx, gy = w.gy, gz = w.gz;
@@ -230,6 +251,13 @@ void p2g_pair(Sim &s, int mp, int n) {
   s.f[3 * n + 0] += sx * sy * sz * mpm * s.g[0];
   s.f[3 * n + 1] += sx * sy * sz * mpm * s.g[1];
   s.f[3 * n + 2] += sx * sy * sz * mpm * s.g[2];
+  // thermal (Extrapolation.cpp:281-283): heat content and the conduction term, with the
+  // particle's temperature gradient of the previous G2P; no periodic pairing of these
+  const double cp = s.mats[mid].specific_heat, kt = s.mats[mid].thermal_conductivity;
+  const double *tg = &s.Tgrad[3 * mp];
+  s.nTmass[n] += sx * sy * sz * mpm * cp;
+  s.nT[n] += sx * sy * sz * s.T[mp] * mpm * cp;
+  s.nTrate[n] -= (mpm / rho) * kt * ((gx * sy * sz * tg[0]) + (sx * gy * sz * tg[1]) + (sx * sy * gz * tg[2]));
 }
 
 // Node-centric gather over a 4x4x4 window of cells — extrapToNodes, Extrapolation.cpp:31-89.
@@ -423,6 +451,7 @@ void grid_bc(Sim &s, int n) {
 // 288-367, with getGIMPNodes (Extrapolation.cpp:649-682).
 void g2p(Sim &s, int mp) {
   s.nrm[3 * mp + 0] = s.nrm[3 * mp + 1] = s.nrm[3 * mp + 2] = 0.0;
+  s.Tgrad[3 * mp + 0] = s.Tgrad[3 * mp + 1] = s.Tgrad[3 * mp + 2] = 0.0;  // Extrapolation.cpp:130-132
   const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
   const int bx = (int)((s.x[3 * mp + 0] - (s.psep / 2) - s.npos[0]) / s.h);
   const int by = (int)((s.x[3 * mp + 1] - (s.psep / 2) - s.npos[1]) / s.h);
@@ -437,6 +466,11 @@ void g2p(Sim &s, int mp) {
         if (s.m[n] == 0.0) continue;
         const Shape w = shape(s, mp, n);
         const double sx = w.sx, sy = w.sy, sz = w.sz, gx = w.gx, gy = w.gy, gz = w.gz;
+        // thermal, at every node with mass whatever its body slots (Extrapolation.cpp:359-362)
+        s.Tgrad[3 * mp + 0] += (gx * sy * sz) * s.nT[n];
+        s.Tgrad[3 * mp + 1] += (sx * gy * sz) * s.nT[n];
+        s.Tgrad[3 * mp + 2] += (sx * sy * gz) * s.nT[n];
+        s.Trate[mp] += sx * sy * sz * s.nTrate[n];
         const int ob = s.obj[mp];
         const double *vn, *an, *nn;
         if (s.body[2 * n + 0] == ob) { vn = &s.vb1[3 * n]; an = &s.ab1[3 * n]; nn = &s.nb1[3 * n]; }
@@ -836,6 +870,7 @@ void update_particle(Sim &s, int mp) {
     for (int k = 0; k < dim; k++) s.x[3 * mp + k] += s.gvel[3 * mp + k] * dt;
   }
   for (int k = 0; k < 3; k++) s.u[3 * mp + k] = s.x[3 * mp + k] - s.x0[3 * mp + k];
+  s.T[mp] += s.Trate[mp] * dt;  // TimeIntegration.cpp:50
   for (int k = 0; k < 6; k++) s.eps[6 * mp + k] += s.erate[6 * mp + k] * dt;
   const gy_material &M = s.mats[s.mat[mp]];
   if (!M.is_rigid) {
@@ -886,6 +921,7 @@ void iterate(Sim &s, double t, int step) {
     s.drate[mp] = 0.0;
     for (int k = 0; k < 3; k++) s.acc[3 * mp + k] = s.gvel[3 * mp + k] = 0.0;
     for (int k = 0; k < 6; k++) s.erate[6 * mp + k] = 0.0;
+    s.Trate[mp] = 0.0;  // TimeIntegration.cpp:160
   }
 #pragma omp parallel for schedule(dynamic, chN)
   for (int n = 0; n < N; n++) gather_node(s, n);
@@ -937,6 +973,7 @@ Sim *create(const gy_grid *gr, const gy_material *mats, int nmat, const gy_param
   s->nbcVal.assign(4 * (size_t)N, 0.0);
   s->cellCount.assign(N, 0);
   s->cellStart.assign(N, 0);
+  for (auto *vec : {&s->nT, &s->nTrate, &s->nTmass}) vec->assign(N, 0.0);
 
   const int P = in->n;
   s->np = P;
@@ -956,6 +993,10 @@ Sim *create(const gy_grid *gr, const gy_material *mats, int nmat, const gy_param
   s->dmg = in->damage ? std::vector<double>(in->damage, in->damage + P) : std::vector<double>(P, 0.0);
   s->smax = in->smax ? std::vector<double>(in->smax, in->smax + P) : std::vector<double>(P, 0.0);
   s->drate.assign(P, 0.0);
+  s->T = in->temperature ? std::vector<double>(in->temperature, in->temperature + P) : std::vector<double>(P, 0.0);
+  s->Tgrad.assign(3 * (size_t)P, 0.0);
+  s->Tgrad_tmp.assign(3 * (size_t)P, 0.0);
+  s->Trate.assign(P, 0.0);
   s->mat = in->material ? std::vector<int>(in->material, in->material + P) : std::vector<int>(P, 0);
   s->obj = in->obj_id ? std::vector<int>(in->obj_id, in->obj_id + P) : std::vector<int>(P, 0);
   s->hold.assign(P, 0);
@@ -1031,6 +1072,7 @@ int32_t cref_set_node_bc(void *h, int32_t n, const int32_t *type, const double *
 
 double cref_dt(void *h) { return ((Sim *)h)->dt; }
 void cref_set_sort_every(void *h, int32_t k) { ((Sim *)h)->sort_every = k > 0 ? k : 10; }
+void cref_set_tgrad_as_reference(void *h, int32_t on) { ((Sim *)h)->tgrad_as_reference = on != 0; }
 void cref_set_dt(void *h, double dt) { ((Sim *)h)->dt = dt; }
 int32_t cref_num_nodes(void *h) { return ((Sim *)h)->numNodes; }
 
@@ -1060,6 +1102,7 @@ void cref_get(void *h, gy_particles *o) {
   put(o->volume, s->vol, s->pid, 1);
   put(o->displacement, s->u, s->pid, 3);
   put(o->cell_id, s->cell, s->pid, 1);
+  put(o->temperature, s->T, s->pid, 1);
 }
 
 // Node fields after the last step: COM mass, COM velocity, body-1 velocity/acceleration,

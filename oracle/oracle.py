@@ -83,9 +83,12 @@ OUT_FIELDS = ["pos", "vel", "stress", "strain", "plastic_strain", "acc_plastic_s
 class CpuSim:
     """One oracle simulation of a `graphyt_amd.workloads.Scene`."""
 
-    def __init__(self, scene, kind: str = "restatement", sort_every: int | None = None):
-        """sort_every (restatement only): re-bin every k steps instead of the reference's
-        cadence (steps divisible by 10, and step 1); 1 re-bins every step."""
+    def __init__(self, scene, kind: str = "restatement", sort_every: int | None = None,
+                 tgrad_moves: bool = False):
+        """Test knobs of the restatement (not the reference): sort_every re-bins every k
+        steps instead of the reference's cadence (steps divisible by 10, and step 1), 1
+        every step; tgrad_moves carries each particle's temperature gradient with it at a
+        sort, where the reference scrambles it (cpu_ref.cpp sort_by_cell)."""
         self.kind = kind
         if kind == "reference":
             # The reference's dynamic OpenMP chunk, P/(16*threads), is evaluated inside the
@@ -111,11 +114,15 @@ class CpuSim:
             rc = self._f("set_node_bc")(self.h, len(t), t.ctypes.data_as(C.POINTER(C.c_int32)), _dp(v))
             if rc != 0:
                 raise RuntimeError("oracle set_node_bc failed")
-        if sort_every is not None:
+        if sort_every is not None or tgrad_moves:
             if kind != "restatement":
-                raise ValueError("sort_every is a knob of the restatement only")
+                raise ValueError("sort_every and tgrad_moves are knobs of the restatement only")
+        if sort_every is not None:
             self.lib.cref_set_sort_every.argtypes = [C.c_void_p, C.c_int32]
             self.lib.cref_set_sort_every(self.h, int(sort_every))
+        if tgrad_moves:
+            self.lib.cref_set_tgrad_as_reference.argtypes = [C.c_void_p, C.c_int32]
+            self.lib.cref_set_tgrad_as_reference(self.h, 0)
 
     def _f(self, name):
         return getattr(self.lib, self.pre + name)

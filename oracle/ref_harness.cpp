@@ -85,6 +85,7 @@ void *rref_create(const gy_grid *g, const gy_material *mats, int32_t nmat, const
     if (in->obj_id) r->mp->setObjectID(i, in->obj_id[i]);
     if (in->hold && in->hold[i]) r->mp->holdParticle(i);
     if (in->damage) r->mp->setDamage(i, in->damage[i]);
+    if (in->temperature) r->mp->setTemp(i, in->temperature[i]);
     if (in->bc_type && in->bc_type[i] != 0) {
       std::vector<double> v(4, 0.0);
       if (in->bc_val)
@@ -186,6 +187,7 @@ void rref_get(void *h, gy_particles *o) {
     if (o->obj_id) o->obj_id[i] = m->objID[p];
     if (o->volume) o->volume[i] = m->volume[p];
     if (o->cell_id) o->cell_id[i] = m->cellID[p];
+    if (o->temperature) o->temperature[i] = m->temperature[p];
   }
 }
 

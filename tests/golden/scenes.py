@@ -14,6 +14,8 @@ a few dozen steps each), one per part of the step the fixtures pin:
                    held particles and node BCs, with the time argument set per step
   rigid_platen     deformable block pressed by a rigid platen (two objects)
   two_blocks_mu    two deformable halves in contact, sliding, friction coefficient 0.3
+  thermal          the elastic cube with a temperature gradient along x (heat transfer:
+                   Extrapolation.cpp:281-283, 359-362, TimeIntegration.cpp:50)
 
 and the plane (2D, Parameters is3D = False) path, per unit depth:
 
@@ -26,6 +28,8 @@ and the plane (2D, Parameters is3D = False) path, per unit depth:
                         partly and fully damaged particles
   plane_brazilian       Drucker-Prager disk with damage between two rigid platens
                         (brazillian2D.py), three objects
+  plane_thermal         a centred elastic square with a temperature gradient (conductivity
+                        0.01: the reference's plane heat update is unstable at copper's 391)
 
 `build(name)` returns (scene, steps, t_of); t_of(step) is the time passed to each
 iterate call, or None for step * dt.
@@ -38,13 +42,16 @@ from graphyt_amd import workloads as W
 from graphyt_amd.abi import BC, material
 
 NAMES = ["elastic", "newtonian", "turbulent", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
-         "bcs", "rigid_platen", "two_blocks_mu"]
+         "bcs", "rigid_platen", "two_blocks_mu", "thermal"]
 PLANE_NAMES = ["plane_dam_break", "plane_elastic", "plane_perfect_plastic", "plane_turbulent",
-               "plane_grady_kipp", "plane_brazilian"]
+               "plane_grady_kipp", "plane_brazilian", "plane_thermal"]
 ALL_NAMES = NAMES + PLANE_NAMES
 # scenes whose particles cross the periodic wrap: the reference's sort cadence (a wrapped
 # particle stays listed at the far side until the next sort) shows in their results
 WRAPPING = ["plane_elastic", "plane_perfect_plastic", "plane_grady_kipp"]
+# scenes with heat transfer: at a sort the reference scrambles the temperature gradient
+# (cpu_ref.cpp sort_by_cell), the GPU path carries it with its particle
+THERMAL = ["thermal", "plane_thermal"]
 
 # fields stored per scene (FP64) and compared
 BASE_FIELDS = ["pos", "vel", "stress", "strain", "density", "cell_id"]
@@ -58,6 +65,8 @@ EXTRA_FIELDS = {
     "plane_turbulent": ["damage"],
     "plane_grady_kipp": ["damage", "smax"],
     "plane_brazilian": ["plastic_strain", "acc_plastic_strain", "damage", "smax"],
+    "thermal": ["temperature"],
+    "plane_thermal": ["temperature"],
 }
 
 
@@ -164,6 +173,15 @@ def build(name: str):
         return W.square_2d(steps=60, mat=m), 60, None
     if name == "plane_brazilian":
         return W.brazilian_disk_2d(h=0.004, v=20.0, steps=300), 300, None
+    if name in ("thermal", "plane_thermal"):
+        if name == "thermal":
+            sc = _cube(40)
+        else:
+            m = material(model=1, bulk_mod=1e6, shear_mod=5e5, density=1000.0, thermal_conductivity=0.01)
+            sc = W.square_2d(steps=40, mat=m, centered=True, vx=0.0)
+        x = sc.particles["pos"][:, 0]
+        sc.particles.arrays["temperature"] = 300.0 + 100.0 * (x - x.min()) / (x.max() - x.min())
+        return sc, 40, None
     raise KeyError(name)
 
 

@@ -42,7 +42,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(abi.GyGrid) == 32
     assert C.sizeof(abi.GyMaterial) == 16 + 15 * 8 + 8
     assert C.sizeof(abi.GyParams) == 8 * 8 + 8
-    assert C.sizeof(abi.GyParticles) == 8 + 19 * 8
+    assert C.sizeof(abi.GyParticles) == 8 + 21 * 8
     assert C.sizeof(abi.GyAudit) == 6 * 8 + 2 * 8
 
 

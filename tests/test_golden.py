@@ -109,3 +109,21 @@ def test_binning_every_step_matches_the_reference_cadence(oracle_mod, name):
     else:
         tol = 1e-10 if name in ("grady_kipp",) else 1e-12
         assert max(r.values()) < tol, r
+
+
+@pytest.mark.parametrize("name", scenes.THERMAL)
+def test_thermal_fixture_and_the_gradient_scramble(oracle_mod, name):
+    """Heat transfer changes the temperatures of the fixture; carrying each particle's
+    gradient with it at a sort (the GPU path) instead of the reference's scalar-indexed
+    gather (Extrapolation.cpp:446, 485) changes the temperatures only, never the mechanics
+    (the thermal fields do not feed back)."""
+    fx = load(name)
+    sc, steps, t_of = scenes.build(name)
+    t0 = sc.particles["temperature"]
+    assert np.abs(fx["temperature"] - t0).max() > 0.05
+    sim = oracle_mod.CpuSim(sc, "restatement", tgrad_moves=True)
+    run(sim, steps, t_of)
+    out = sim.get(["pos", "stress", "temperature"])
+    sim.close()
+    assert np.array_equal(out["pos"], fx["pos"]) and np.array_equal(out["stress"], fx["stress"])
+    assert not np.array_equal(out["temperature"], fx["temperature"])

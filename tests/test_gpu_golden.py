@@ -6,7 +6,9 @@ reference accumulates rounding per step, and plastic and damage branches widen i
 Scenes whose particles cross the periodic wrap (scenes.WRAPPING) are compared with the
 oracle re-binning every step, as the GPU path does: there the reference's sort cadence
 drops a wrapped particle's P2G contributions until its next sort (test_golden.py pins the
-oracle to the fixture at the reference's cadence and shows the difference).
+oracle to the fixture at the reference's cadence and shows the difference). Scenes with
+heat transfer (scenes.THERMAL) are compared with the oracle carrying each particle's
+temperature gradient with it at a sort, where the reference scrambles it.
 """
 import os
 import sys
@@ -38,6 +40,8 @@ TOL = {
     "plane_perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
     "plane_turbulent": ELASTIC,
     "plane_grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
+    "thermal": dict(ELASTIC, temperature=1e-5),
+    "plane_thermal": dict(ELASTIC, temperature=1e-5),
 }
 # plane scenes the GPU path runs (the plane Brazilian disk needs contact between objects)
 PLANE_GPU = [n for n in scenes.PLANE_NAMES if n != "plane_brazilian"]
@@ -60,8 +64,8 @@ def gpu_run(name, oracle_mod=None):
     with np.load(os.path.join(HERE, "golden", f"{name}.npz")) as f:
         fx = {k: f[k] for k in f.files}
     assert np.allclose(scenes.input_digest(sc), fx["digest"], rtol=1e-12, atol=0), "scene builder changed"
-    if name in scenes.WRAPPING:
-        sim = oracle_mod.CpuSim(sc, "restatement", sort_every=1)
+    if name in scenes.WRAPPING or name in scenes.THERMAL:
+        sim = oracle_mod.CpuSim(sc, "restatement", sort_every=1, tgrad_moves=name in scenes.THERMAL)
         for s in range(steps):
             sim.iterate(s * sim.dt if t_of is None else t_of(s), s)
         fx = sim.get(scenes.fields(name))

@@ -82,3 +82,44 @@ def test_graph_replayed_steps_match_python_loop():
     s2.iterate_GPU_steps(40, prm2)
     assert rel(mp2.getPosArr(), mp1.getPosArr()) <= 1e-7
     assert rel(mp2.getStressArr(), mp1.getStressArr()) <= 1e-4
+
+
+def test_plane_dam_break_script_matches_reference_fixture():
+    """Example_Scripts/dambreak_2D.py written against the graphyt module (Parameters(tmax,
+    False), L[2] = 0, wall nodes held by grid_vx_vy), stepped with iterate_GPU: the plane
+    path, compared with the fixture the reference wrote for the same scene."""
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import scenes
+    sc, steps, _ = scenes.build("plane_dam_break")
+    nodes = graphyt.Nodes(list(sc.L), sc.h)
+    assert nodes.Nz == 1 and nodes.numNodes == sc.num_nodes
+    mp = graphyt.MaterialPoints(nodes)
+    mp.psep = sc.psep
+    for i, p in enumerate(sc.particles["pos"]):
+        mp.add(i, *p)
+    water = graphyt.MaterialModels()
+    water.setBulkMod(1e6)
+    water.setShearMod(0.0)
+    water.setDensity(1000)
+    water.setModel(2)
+    wall = 2.0 * nodes.cellsize
+    for n in range(nodes.numNodes):
+        x, y = nodes.getPosX(n), nodes.getPosY(n)
+        if x >= sc.L[0] - wall or x <= wall or y >= sc.L[1] - wall or y <= wall:
+            nodes.setBC(n, graphyt.BCTypes.grid_vx_vy, [0, 0])
+    prm = graphyt.Parameters(5.0, False)
+    prm.setGravity(0, -10, 0)
+    prm.setDampingCoef(0.5)
+    solver = graphyt.Solver(nodes, mp, prm, [water])
+    solver.setDt(sc.resolved_dt())
+    assert mp.getMassArr()[0] == pytest.approx(1000.0 * sc.psep ** 2)  # per unit depth
+    for step in range(steps):
+        solver.iterate_GPU(step * solver.dt, step, prm)
+    with np.load(os.path.join(here, "golden", "plane_dam_break.npz")) as f:
+        fx = {k: f[k] for k in f.files}
+    assert rel(mp.getPosArr().reshape(-1, 3), fx["pos"]) <= 1e-6
+    assert rel(mp.getVelArr().reshape(-1, 3), fx["vel"]) <= 1e-4
+    assert rel(mp.getStressArr().reshape(-1, 6), fx["stress"]) <= 1e-4
