@@ -41,13 +41,14 @@ B_ALG_G2P2G = B_ALG_STEP - 8
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "g2p2g_dp_column_1M.md")
+PLANE_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "g2p2g_plane_dam_break_0.94M.md")
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, profile: str = TRAFFIC_PROFILE):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration), or None."""
     try:
-        text = open(TRAFFIC_PROFILE).read()
+        text = open(profile).read()
         blob = text.split("```json")[-1].split("```")[0]
         rec = json.loads(blob)
         return int(rec["traffic_bytes"]) if rec.get("kernel") == kernel else None
@@ -59,13 +60,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_steps: int, warm: int) -> dict:
+def cpu_baseline(sample_steps: int, warm: int, workload: str = "dp_column") -> dict:
     """Time the CPU reference on the same workload (run in a child process)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from graphyt_amd import workloads as W
     kind = "reference" if oracle.available("reference") else "restatement"
-    sc = W.dp_column()
+    sc = plane_scene() if workload == "plane_dam_break" else W.dp_column()
     sim = oracle.CpuSim(sc, kind)
     for s in range(warm):
         sim.iterate(s * sim.dt, s)
@@ -76,7 +77,7 @@ def cpu_baseline(sample_steps: int, warm: int) -> dict:
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": sc.n * sample_steps / dt, "unit": "particle-steps/s", "cores": cores,
             "kind": "reference" if kind == "reference" else "port",
-            "sample": f"dp_column 1,024,000 particles, steps {warm}..{warm + sample_steps - 1} "
+            "sample": f"{workload} {sc.n:,} particles, steps {warm}..{warm + sample_steps - 1} "
                       f"({sample_steps} steps incl. the every-10-steps sort), FP64, "
                       f"OMP_NUM_THREADS={cores}, {dt:.1f} s"}
 
@@ -195,6 +196,78 @@ def bench_gk_disk(args) -> int:
     return 0
 
 
+# Plane dam break (Newtonian, single body): the fused plane kernel reads and writes 22
+# state words per particle (q 3, v 3, sigma 6, eps 6, rho, m, tag, pid: 176 B), reads its
+# list entry and writes key and rank (12 B), and per tile of 8 x 8 cells (256 particles when
+# full) stages 100 node velocities and writes 100 node partials (32 B each): 25 B per
+# particle. The step adds place (12 B) and the grid update (about 1/4 node per particle,
+# partial reads and a VEL write: 20 B).
+B_ALG_PLANE_G2P2G = 176 + 12 + 25
+B_ALG_PLANE_STEP = B_ALG_PLANE_G2P2G + 12 + 20
+
+
+def plane_scene():
+    from graphyt_amd import workloads as W
+    return W.dam_break_2d(h=0.00125)
+
+
+def bench_plane(args) -> int:
+    """The plane (is3D = false) path at about a million particles: the dam break of
+    Example_Scripts/dambreak_2D.py at h = 1.25 mm (801 x 801 nodes, 944,860 particles)."""
+    from graphyt_amd import runtime
+    sc = plane_scene()
+    log(f"workload plane_dam_break: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
+    ctx = runtime.Context(sc, graph=True)
+    ctx.run(args.warmup)
+    ctx.synchronize()
+    ctx.set_option(runtime.OPT_PROFILE, 1)
+    ctx.reset_kernel_stats()
+    t0 = time.perf_counter()
+    ctx.run(args.steps, step0=args.warmup)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.set_option(runtime.OPT_PROFILE, 0)
+    t0 = time.perf_counter()
+    ctx.run(args.steps, step0=args.warmup + args.steps)
+    ctx.synchronize()
+    wall_graph = time.perf_counter() - t0
+    audit = ctx.audit()
+    log("kernel stats (launches, total ms):", stats)
+    best = sc.n * args.steps / min(wall, wall_graph)
+    launches, total_ms = stats["g2p2g"]
+    avg_s = total_ms / launches / 1e3
+    bpl = B_ALG_PLANE_G2P2G * sc.n
+    achieved = bpl / avg_s / 1e9
+    cpu = None
+    if not args.no_cpu:
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload",
+                                "plane_dam_break", "--cpu-steps", str(min(args.cpu_steps, 30))], capture_output=True,
+                               text=True, timeout=900)
+            cpu = json.loads(r.stdout.strip().splitlines()[-1])
+        except Exception as e:
+            log("cpu baseline failed:", e)
+    out = {
+        "metric": "particle-steps/s (plane GIMP, 0.94M pts)", "value": round(best, 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * sc.n / best, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plane_dam_break: 2D Newtonian dam break (dambreak_2D.py), h 1.25 mm",
+                   "particles": sc.n, "nodes": list(sc.dims), "dt": sc.resolved_dt(), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "g2p2g", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("g2p2g", PLANE_TRAFFIC_PROFILE),
+                     "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bpl},
+        "cpu_baseline": cpu,
+        "step_hbm_frac": round(B_ALG_PLANE_STEP * best / (HBM_PEAK_GBS * 1e9), 4),
+        "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in stats.items()},
+        "audit_nonfinite": audit["nonfinite"],
+    }
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
 def bench_surrogate(args) -> int:
     """The learned return map against the exact FP32 device map on the column's constitutive
     samples: 20 pairs of consecutive 1,024,000-particle states (20.5M samples) train the
@@ -242,7 +315,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
-    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate", "gk_disk"], default="dp_column",
+    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate", "gk_disk", "plane_dam_break"],
+                    default="dp_column",
                     help="dp_column: the 1M-particle column (default); ensemble: 256 batched impacts; "
                          "surrogate: learned vs exact Drucker-Prager return map")
     ap.add_argument("--autotune", action="store_true", help="run gy_autotune before timing")
@@ -254,7 +328,7 @@ def main():
         return copy_calibration()
 
     if args.cpu_baseline_only:
-        print(json.dumps(cpu_baseline(args.cpu_steps, 2)))
+        print(json.dumps(cpu_baseline(args.cpu_steps, 2, args.workload)))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the MPM step on one GPU; multi-GPU runs are not "
@@ -268,6 +342,8 @@ def main():
         return bench_surrogate(args)
     if args.workload == "gk_disk":
         return bench_gk_disk(args)
+    if args.workload == "plane_dam_break":
+        return bench_plane(args)
     sc = W.dp_column()
     log(f"workload dp_column: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
     ctx = runtime.Context(sc, graph=True)

@@ -46,8 +46,8 @@ __device__ __forceinline__ long long plane_canon(const InstDesc &I, int g0, int 
   return I.node_base + c0 + (long long)I.n[0] * c1;
 }
 
-__device__ __forceinline__ void plane_flag(const DevPtrs &d, const InstDesc &I, int c0, int c1) {
-  d.brick_flag[I.brick_base + c0 / PB + I.nb[0] * (c1 / PB)] = 1;
+__device__ __forceinline__ void plane_flag(uint8_t *brick_flag, const InstDesc &I, int c0, int c1) {
+  brick_flag[I.brick_base + c0 / PB + I.nb[0] * (c1 / PB)] = 1;
 }
 
 // Keys + histogram of the current buffer (first step after an upload).
@@ -192,8 +192,10 @@ struct PlaneRec {
 constexpr int PACC_N = PR_N;
 
 // P2G of one particle straight into the global accumulators (its stencil left the tile's
-// LDS region, e.g. after a periodic wrap).
-__device__ __noinline__ void plane_p2g_global(const DevPtrs &d, const InstDesc *Ip, int b0, int b1, Axis W0, Axis W1,
+// LDS region, e.g. after a periodic wrap). Out of line, with plain pointer arguments: a
+// reference to the kernel's DevPtrs argument would make every lane keep a private copy.
+__device__ __noinline__ void plane_p2g_global(float *acc, uint8_t *brick_flag, const InstDesc *Ip, int b0, int b1,
+                                              Axis W0, Axis W1,
                                               float m, float mv0, float mv1, float mv2, float vs0, float vs1,
                                               float vs3, float vs4, float vs5) {
   const InstDesc &I = *Ip;
@@ -209,8 +211,8 @@ __device__ __noinline__ void plane_p2g_global(const DevPtrs &d, const InstDesc *
       int c0, c1;
       const long long ci = plane_canon(I, b0 + i, b1 + j, c0, c1);
 #pragma unroll
-      for (int f = 0; f < 7; f++) unsafeAtomicAdd(&d.acc[ci * NODE_W + f], t[f]);
-      plane_flag(d, I, c0, c1);
+      for (int f = 0; f < 7; f++) unsafeAtomicAdd(&acc[ci * NODE_W + f], t[f]);
+      plane_flag(brick_flag, I, c0, c1);
     }
 }
 
@@ -415,7 +417,7 @@ __device__ __forceinline__ void plane_scatter(const DevPtrs &d, const InstDesc &
   const float vs0 = w1.z, vs1 = w1.w, vs3 = w2.x, vs4 = w2.y, vs5 = w2.z;
   const int r0 = b0 - o0 + 1, r1 = b1 - o1 + 1;
   if (!(r0 >= 0 && r1 >= 0 && r0 + 2 < PR && r1 + 2 < PR)) {
-    plane_p2g_global(d, &d.inst[inst], b0, b1, W0, W1, m, mv0, mv1, mv2, vs0, vs1, vs3, vs4, vs5);
+    plane_p2g_global(d.acc, d.brick_flag, &d.inst[inst], b0, b1, W0, W1, m, mv0, mv1, mv2, vs0, vs1, vs3, vs4, vs5);
     return;
   }
 #pragma unroll
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       if (g0 < 0 || g1 < 0 || g0 >= I.n[0] || g1 >= I.n[1]) continue;
       int cc0, cc1;
       (void)plane_canon(I, g0, g1, cc0, cc1);
-      plane_flag(d, I, cc0, cc1);
+      plane_flag(d.brick_flag, I, cc0, cc1);
     }
     if (__syncthreads_or(halo)) {
       float *pt = d.partial + (long long)t * PARTIAL_W;
