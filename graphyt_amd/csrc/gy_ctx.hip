@@ -339,6 +339,19 @@ int check_status(gy_ctx *c) {
 void enqueue_step(gy_ctx *c) {
   void *s = c->stream;
   const bool th = c->d.thermal != 0;
+  if (c->plane && c->d.multibody) {
+    timed(c, K_GRID, [&] { launch_plane_mb_grid(c->d, s); });
+    if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
+    timed(c, K_G2P2G, [&] { launch_plane_mb_update(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
+    timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    timed(c, K_MB_CLAIM, [&] { launch_mb_cells(c->d, s); launch_plane_mb_claim(c->d, s); });
+    timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
+    timed(c, K_MB_P2G, [&] { launch_plane_mb_p2g(c->d, s); });
+    if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
+    return;
+  }
   if (c->plane) {
     timed(c, K_GRID, [&] { launch_plane_grid(c->d, s); });
     if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
@@ -379,6 +392,13 @@ void enqueue_init(gy_ctx *c) {
     timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s); });
     timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
     timed(c, K_PLACE, [&] { launch_place(c->d, s); });
+    if (c->d.multibody) {
+      timed(c, K_MB_CLAIM, [&] { launch_mb_cells(c->d, s); launch_plane_mb_claim(c->d, s); });
+      timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
+      timed(c, K_MB_P2G, [&] { launch_plane_mb_p2g(c->d, s); });
+      if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
+      return;
+    }
     timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
     launch_brick_list(c->d, s);
     if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
@@ -630,8 +650,6 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     I.pcount = n;
     pstart += n;
   }
-  if (multibody && c->plane)
-    return fail(c, GY_ERR_UNSUPPORTED, "plane (is3D = false) contexts hold one deformable object (no contact yet)");
   d.kmask = kmask;
   d.multibody = multibody ? 1 : 0;
   d.thermal = thermal ? 1 : 0;

@@ -179,5 +179,127 @@ struct TileQueue {
   }
 };
 
+// calculateColoumbFrictionContactNormals (Grid.cpp:247-308) for one body: the contact
+// normal n + min(mu, mu0) * (omega x ...) with the reference's component formulas.
+__device__ __forceinline__ void contact_normal(const float rel[3], const float n[3], float mu, float cn[3]) {
+  float om[3] = {rel[1] * n[2] - rel[2] * n[1], rel[0] * n[2] - rel[2] * n[0], rel[0] * n[1] - rel[1] * n[0]};
+  const float omag = sqrtf(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]) + 0.001f;
+  for (int a = 0; a < 3; a++) om[a] /= omag;
+  const float mu0 = omag / (rel[0] * n[0] + rel[1] * n[1] + rel[2] * n[2]) + 0.001f;
+  const float on[3] = {om[2] * n[1] - om[1] * n[2], om[2] * n[0] - om[0] * n[2], om[1] * n[0] - om[0] * n[1]};
+  const float m = fminf(mu, mu0);
+  for (int a = 0; a < 3; a++) cn[a] = n[a] + m * on[a];
+}
+
+// Contact solve of one actual node (Grid.cpp:9-245) and its BCs on the centre of mass and
+// both bodies (Grid.cpp:967-982, 1012-1031): m, a7 the centre-of-mass sums of its periodic
+// class (m, p, f_int), ab the body sums [m, p, f_int, rigid mass] x 2, vc and ac the
+// centre-of-mass velocity p/m and acceleration f/m + g. Writes VEL (centre of mass) and
+// VELB (body velocities and accelerations) of the node and clears its normal sums.
+__device__ __forceinline__ void mb_solve_node(const DevPtrs &d, const InstDesc &I, long long ni, float m,
+                                              const float a7[7], const float ab[MB_ACC2_W], const float vc[3],
+                                              const float ac[3], float dt, float kick) {
+  // body sums: [m, p, f_int, rigid mass]; gravity only on the non-rigid mass
+  float mb[2], pb[2][3], fb[2][3];
+  auto body_sums = [&]() {
+    for (int s2 = 0; s2 < 2; s2++) {
+      mb[s2] = ab[8 * s2];
+      for (int k = 0; k < 3; k++) {
+        pb[s2][k] = ab[8 * s2 + 1 + k];
+        fb[s2][k] = ab[8 * s2 + 4 + k] + (ab[8 * s2] - ab[8 * s2 + 7]) * I.gravity[k];
+      }
+    }
+  };
+  const int4 cl = d.claim[ni];
+  body_sums();
+  float nr[2][3];
+  for (int k = 0; k < 3; k++) {
+    nr[0][k] = d.nrm[ni * 6 + k];
+    nr[1][k] = d.nrm[ni * 6 + 3 + k];
+    d.nrm[ni * 6 + k] = 0.f;
+    d.nrm[ni * 6 + 3 + k] = 0.f;
+  }
+  float vb[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, acb[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  if (cl.y == -1) {  // one body: its fields are the centre-of-mass sums
+    mb[0] = m;
+    for (int k = 0; k < 3; k++) {
+      pb[0][k] = a7[1 + k];
+      fb[0][k] = a7[4 + k] + (m - ab[7]) * I.gravity[k];
+    }
+  }
+  if (m != 0.0f) {
+    for (int s2 = 0; s2 < 2; s2++) {  // normalise (Grid.cpp:19-43)
+      const float mag = sqrtf(nr[s2][0] * nr[s2][0] + nr[s2][1] * nr[s2][1] + nr[s2][2] * nr[s2][2]);
+      for (int k = 0; k < 3; k++) nr[s2][k] = (mag > 1e-8f) ? nr[s2][k] / mag : 0.f;
+    }
+    for (int k = 0; k < 3; k++) nr[1][k] = -nr[0][k];  // Grid.cpp:45-47
+    auto free_body = [&](int s2) {
+      if (mb[s2] == 0.0f) return;
+      for (int k = 0; k < 3; k++) {
+        acb[s2][k] = fb[s2][k] / mb[s2];
+        vb[s2][k] = pb[s2][k] / mb[s2] + acb[s2][k] * kick;
+      }
+    };
+    if (cl.y != -1) {
+      float rel[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, ap[2] = {0.f, 0.f};
+      for (int s2 = 0; s2 < 2; s2++)
+        if (mb[s2] != 0.0f) {
+          for (int k = 0; k < 3; k++) rel[s2][k] = pb[s2][k] / mb[s2] - vc[k];
+          ap[s2] = rel[s2][0] * nr[s2][0] + rel[s2][1] * nr[s2][1] + rel[s2][2] * nr[s2][2];
+        }
+      if (ap[0] > 0.01f || ap[1] > 0.01f) {
+        if (cl.z || cl.w) {  // a rigid body: the other takes its velocity (Grid.cpp:78-105)
+          const int r = cl.z ? 0 : 1, o2 = 1 - r;
+          if (mb[r] != 0.0f) {
+            for (int k = 0; k < 3; k++) vb[r][k] = pb[r][k] / mb[r];
+            if (mb[o2] != 0.0f)
+              for (int k = 0; k < 3; k++)
+                vb[o2][k] = pb[o2][k] / mb[o2] - (pb[o2][k] - mb[o2] * vb[r][k]) / mb[o2];
+          }
+        } else {  // two deformable bodies (Grid.cpp:106-170)
+          for (int s2 = 0; s2 < 2; s2++) {
+            if (mb[s2] == 0.0f) continue;
+            float cn[3];
+            contact_normal(rel[s2], nr[s2], I.mu, cn);
+            for (int k = 0; k < 3; k++) {
+              vb[s2][k] = pb[s2][k] / mb[s2] - ap[s2] * cn[k];
+              acb[s2][k] = fb[s2][k] / mb[s2] - (ap[s2] * cn[k]) / dt;
+              vb[s2][k] += acb[s2][k] * kick;
+            }
+          }
+        }
+      } else {
+        free_body(0);
+        free_body(1);
+      }
+    } else {
+      free_body(0);
+    }
+  }
+  float vv[3] = {vc[0], vc[1], vc[2]}, aa[3] = {ac[0], ac[1], ac[2]};
+  const int t = d.bc_type[ni];
+  if (t == 1 || t == 2 || t == 11) {  // Grid.cpp:967-982, 1012-1031
+    const float b0 = d.bc_val[2 * ni], b1v = d.bc_val[2 * ni + 1];
+    const int ax0 = (t == 2) ? 1 : 0;
+    const float val0 = b0;
+    vv[ax0] = val0; aa[ax0] = 0.f;
+    for (int s2 = 0; s2 < 2; s2++) { vb[s2][ax0] = val0; acb[s2][ax0] = 0.f; }
+    if (t == 11) {
+      vv[1] = b1v; aa[1] = 0.f;
+      for (int s2 = 0; s2 < 2; s2++) { vb[s2][1] = b1v; acb[s2][1] = 0.f; }
+    }
+  }
+  float4 *vp = reinterpret_cast<float4 *>(d.vel + ni * NODE_W);
+  vp[0] = make_float4(vv[0], vv[1], vv[2], aa[0]);
+  vp[1] = make_float4(aa[1], aa[2], m, 0.f);
+  float *vq = d.velb + ni * MB_VELB_W;
+  for (int k = 0; k < 3; k++) {
+    vq[k] = vb[0][k];
+    vq[3 + k] = acb[0][k];
+    vq[6 + k] = vb[1][k];
+    vq[9 + k] = acb[1][k];
+  }
+}
+
 }  // namespace
 }  // namespace gy

@@ -238,6 +238,13 @@ void launch_plane_keys(const DevPtrs &d, void *stream);
 void launch_plane_p2g_only(const DevPtrs &d, void *stream);
 void launch_plane_g2p2g(const DevPtrs &d, void *stream);
 void launch_plane_grid(const DevPtrs &d, void *stream);
+// plane contact step: plane_mb_grid, plane_mb_update, scan_partial (+ advance), scan_final,
+// place, mb_cells, plane_mb_claim, plane_p2g_only, plane_mb_p2g
+void launch_mb_cells(const DevPtrs &d, void *stream);  // objects per cell (both paths)
+void launch_plane_mb_claim(const DevPtrs &d, void *stream);
+void launch_plane_mb_p2g(const DevPtrs &d, void *stream);
+void launch_plane_mb_grid(const DevPtrs &d, void *stream);
+void launch_plane_mb_update(const DevPtrs &d, void *stream);
 // heat transfer (gy_thermal.hip): node sums of the current state (cleared first), and the
 // particle gradient, rate and temperature update before the fused particle kernel
 void launch_therm_p2g(const DevPtrs &d, void *stream);

@@ -219,10 +219,15 @@ __device__ __noinline__ void plane_p2g_global(float *acc, uint8_t *brick_flag, c
 // Phase A for list entry i (slot `slot` of the source buffer): G2P + update + models +
 // store at sorted position i (UPDATE), and the P2G record of the (new) state into column
 // `col` of s_rec. Returns the new key (UPDATE) or -1.
-template <bool UPDATE, int MASK>
+// MB (contact between objects): G2P reads the velocity field of the body slot the
+// particle's object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots'
+// objects; Extrapolation.cpp:288-367), and no record is written (P2G runs after the
+// re-binning).
+template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I, int o0, int o1, double tl, int i,
                                             int slot, const float4 (*s_vel)[2], const MatDev *s_mat, int par,
-                                            PlaneRec *s_rec, int col) {
+                                            PlaneRec *s_rec, int col, const float4 (*s_vel2)[2] = nullptr,
+                                            const int (*s_obj)[PG_N] = nullptr) {
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
   const long long cap = d.cap;
@@ -260,7 +265,16 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
       const int row = r0 + PG_R * (r1 + jj);
 #pragma unroll
       for (int ii = 0; ii < 3; ii++) {
-        const float4 a0 = s_vel[row + ii][0], a1 = s_vel[row + ii][1];
+        float4 a0 = s_vel[row + ii][0], a1 = s_vel[row + ii][1];
+        if (MB) {
+          const int n = row + ii, ob = tag_obj(tag);
+          const int sl = (s_obj[0][n] == ob) ? 0 : (s_obj[1][n] == ob) ? 1 : 2;
+          const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (sl != 0) {
+            a0 = (sl == 1) ? s_vel2[n][0] : z4;
+            a1 = (sl == 1) ? s_vel2[n][1] : z4;
+          }
+        }
         const float vn[3] = {a0.x, a0.y, a0.z}, an[3] = {a0.w, a1.x, a1.y};
         const float wx = W0.w[ii], gx = W0.g[ii];
 #pragma unroll
@@ -391,6 +405,7 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
     if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
   }
 
+  if (MB) return knew;
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
@@ -547,6 +562,204 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Contact between objects in a plane (the plane counterparts of the 3D k_mb_* kernels).
+
+// Body claims of every node a particle of an active tile can reach (the tile's 10 x 10
+// core): the node's window is the 4 x 4 cells nidx-2..nidx+1 with linear cell ids and only
+// a range check (Extrapolation.cpp:89-121), met in ascending (j, i); per cell its objects
+// (k_mb_cells) in ascending object id.
+__global__ __launch_bounds__(WG) void k_plane_mb_claim(DevPtrs d) {
+  const long long nact = d.ctrl->active_tiles;
+  for (long long it = (long long)blockIdx.x * WG + threadIdx.x; it < nact * PC_N;
+       it += (long long)gridDim.x * WG) {
+    const int t = d.active[it / PC_N], ln = (int)(it % PC_N);
+    const InstDesc &I = d.inst[d.tile_inst[t]];
+    const int lt = t - I.tile_base;
+    const int g0 = (lt % I.nt[0]) * PT + ln % PC, g1 = (lt / I.nt[0]) * PT + ln / PC;
+    if (g0 >= I.n[0] || g1 >= I.n[1]) continue;
+    const int Cx = I.n[0] - 1, Cy = I.n[1] - 1;
+    const long long ncell = (long long)Cx * Cy;
+    int b1 = -1, b2 = -1, r1 = 0, r2 = 0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int jy = -2; jy < 2; jy++)
+        for (int ix = -2; ix < 2; ix++) {
+          const long long lin = (g0 + ix) + (long long)(g1 + jy) * Cx;
+          if (lin < 0 || lin >= ncell) continue;
+          const int c0 = (int)(lin % Cx), c1 = (int)(lin / Cx);
+          if (c0 > I.n[0] - 3 || c1 > I.n[1] - 3) continue;  // no particle there
+          const int k = plane_key(I, c0, c1);
+          if (d.offsets[k + 1] == d.offsets[k]) continue;
+          const int4 co = d.cellobj[k];
+          if (pass == 0) {
+            const int ob[2] = {co.x, co.y};
+            for (int q = 0; q < 2; q++) {
+              if (ob[q] < 0) continue;
+              if (b1 < 0) b1 = ob[q];
+              else if (b2 < 0 && ob[q] != b1) b2 = ob[q];
+            }
+          } else {
+            r1 |= (co.x == b1 && co.z) || (co.y == b1 && co.w);
+            r2 |= b2 >= 0 && ((co.x == b2 && co.z) || (co.y == b2 && co.w));
+          }
+        }
+    d.claim[I.node_base + g0 + (long long)I.n[0] * g1] = make_int4(b1, b2, r1, r2);
+  }
+}
+
+// Body-field P2G of one particle at the nodes where two bodies are claimed (rigid
+// particles: unit mass, their mass also added to the slot's rigid-mass sum at every node),
+// folded onto the canonical node; normals per actual node with the particle's volume m/rho
+// (Extrapolation.cpp:210-256). FP32 global atomics.
+__global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
+  const int s = blockIdx.x * WG + threadIdx.x;
+  if (s >= d.np) return;
+  const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
+  const long long cap = d.cap;
+  const InstDesc &I = d.inst[inst_of_slot(d, s)];
+  const uint32_t tag = A[F_TAG * cap + s];
+  const MatDev &M = d.mats[I.mat_base + tag_mat(tag)];
+  const int obj = tag_obj(tag);
+  bool bad = false;
+  const int q0 = (int)A[F_QX * cap + s], q1 = (int)A[F_QY * cap + s];
+  const int b0 = stencil_base(q0, I.lpq, I.fbits, I.n[0] - 3, bad);
+  const int b1 = stencil_base(q1, I.lpq, I.fbits, I.n[1] - 3, bad);
+  const Axis W0 = gimp3((float)(q0 - (b0 << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+  const Axis W1 = gimp3((float)(q1 - (b1 << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
+  const float mass = f32(A[F_MASS * cap + s]), rho = f32(A[F_RHO * cap + s]);
+  const float mp = M.rigid ? 1.0f : mass;
+  const float nV = -(mp / rho), Vn = mass / rho;
+  float mv[3], vs[6];
+  for (int a = 0; a < 3; a++) mv[a] = mp * f32(A[(F_VX + a) * cap + s]);
+  for (int a = 0; a < 6; a++) vs[a] = nV * f32(A[(F_S0 + a) * cap + s]);
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const float w = W0.w[i] * W1.w[j], gx = W0.g[i] * W1.w[j], gy = W0.w[i] * W1.g[j];
+      if (w == 0.0f && gx == 0.0f && gy == 0.0f) continue;
+      const long long gi = I.node_base + (b0 + i) + (long long)I.n[0] * (b1 + j);
+      const int4 cl = d.claim[gi];
+      const bool two = cl.y != -1;
+      if (!two && !M.rigid) continue;
+      const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
+      if (sl < 0) continue;
+      int c0, c1;
+      const long long ci = plane_canon(I, b0 + i, b1 + j, c0, c1);
+      float *ab = d.acc2 + ci * MB_ACC2_W + sl * 8;
+      if (M.rigid) unsafeAtomicAdd(&ab[7], w * mp);
+      if (!two) continue;
+      const float t[7] = {w * mp, w * mv[0], w * mv[1], w * mv[2], vs[0] * gx + vs[3] * gy, vs[3] * gx + vs[1] * gy,
+                          vs[5] * gx + vs[4] * gy};
+      for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], t[f]);
+      float *nb = d.nrm + gi * 6 + sl * 3;
+      unsafeAtomicAdd(&nb[0], gx * Vn);
+      unsafeAtomicAdd(&nb[1], gy * Vn);
+    }
+}
+
+// Contact nodal solve on the flagged plane bricks (one wave per 8 x 8 brick): centre-of-mass
+// sums of the periodic class (tile partials and direct scatter), body sums, then per actual
+// node its own claims, normals, solve and BCs (mb_solve_node).
+__global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gb = blockIdx.x * (WG / 64) + wave;
+  if (gb >= d.nbricks) return;
+  if (!d.brick_flag[gb]) return;
+  const InstDesc &I = d.inst[d.brick_inst[gb]];
+  const int lb = gb - I.brick_base;
+  const int c0 = (lb % I.nb[0]) * PB + (lane & 7), c1 = (lb / I.nb[0]) * PB + (lane >> 3);
+  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
+  if (c0 < I.cper[0] && c1 < I.cper[1]) {
+    const long long ci = I.node_base + c0 + (long long)I.n[0] * c1;
+    float a7[7] = {0, 0, 0, 0, 0, 0, 0}, ab[MB_ACC2_W];
+    for (int my = 0; my < 2; my++) {
+      if (my && c1 >= 3) break;
+      for (int mx = 0; mx < 2; mx++) {
+        if (mx && c0 >= 3) break;
+        plane_gather(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], a7);
+      }
+    }
+    for (int f = 0; f < 7; f++) { a7[f] += d.acc[ci * NODE_W + f]; d.acc[ci * NODE_W + f] = 0.f; }
+    for (int f = 0; f < MB_ACC2_W; f++) { ab[f] = d.acc2[ci * MB_ACC2_W + f]; d.acc2[ci * MB_ACC2_W + f] = 0.f; }
+    const float m = a7[0];
+    const float dt = I.dt, kick = (step == 1) ? 0.5f * dt : dt;
+    float vc[3] = {0.f, 0.f, 0.f}, ac[3] = {0.f, 0.f, 0.f};
+    if (m != 0.0f)
+      for (int k = 0; k < 3; k++) {
+        vc[k] = a7[1 + k] / m;
+        ac[k] = (a7[4 + k] + m * I.gravity[k]) / m;
+      }
+    for (int my = 0; my < 2; my++) {
+      if (my && c1 >= 3) break;
+      for (int mx = 0; mx < 2; mx++) {
+        if (mx && c0 >= 3) break;
+        const long long ni = I.node_base + (c0 + mx * I.cper[0]) + (long long)I.n[0] * (c1 + my * I.cper[1]);
+        mb_solve_node(d, I, ni, m, a7, ab, vc, ac, dt, kick);
+      }
+    }
+  }
+  if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// G2P + update + models + store + new key on the body-slot velocity fields, persistent over
+// the active tiles (the plane fused kernel without its P2G phase).
+__global__ __launch_bounds__(WG) void k_plane_mb_update(DevPtrs d) {
+  __shared__ float4 s_vel[PG_N][2], s_vel2[PG_N][2];
+  __shared__ int s_obj[2][PG_N];
+  __shared__ MatDev s_mat[MAXMAT];
+  __shared__ int s_off[TILE3 + 1];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x;
+  const int par = d.ctrl->parity;
+  const TileQueue queue(d);
+  for (;;) {
+    if (tid == 0) s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
+    if (t < 0) break;
+    const int k0 = t * TILE3;
+    const int ii = d.tile_inst[t];
+    const InstDesc I = d.inst[ii];
+    const int lt = t - I.tile_base;
+    const int o0 = (lt % I.nt[0]) * PT, o1 = (lt / I.nt[0]) * PT;
+    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
+    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+    for (int n = tid; n < PG_N; n += WG) {
+      const int g0 = o0 + n % PG_R, g1 = o1 + n / PG_R;
+      float vq[MB_VELB_W];
+      for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
+      int4 cl = make_int4(-1, -1, 0, 0);
+      if (g0 < I.n[0] && g1 < I.n[1]) {
+        const long long ni = I.node_base + g0 + (long long)I.n[0] * g1;
+        if (d.vel[ni * NODE_W + 6] != 0.0f) {  // G2P skips nodes without mass (Extrapolation.cpp:290)
+          for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
+          cl = d.claim[ni];
+        }
+      }
+      s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+      s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
+      s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
+      s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
+      s_obj[0][n] = cl.x;
+      s_obj[1][n] = cl.y;
+    }
+    __syncthreads();
+    const double tl = d.t_last[ii];
+    const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
+    for (int c0 = p0; c0 < p1; c0 += WG) {
+      const int i = c0 + tid;
+      int knew = -1;
+      if (i < p1)
+        knew = plane_update<true, KM_ALL, true>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, s_mat, par, nullptr, 0, s_vel2,
+                                                s_obj);
+      const int r = wave_hist_rank(d.count, knew);
+      if (knew >= 0) d.rank[i] = r;
+    }
+    __syncthreads();
+  }
+}
+
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int blocks(long long n, int per) { return (int)((n + per - 1) / per); }
 
@@ -567,6 +780,19 @@ void launch_plane_g2p2g(const DevPtrs &d, void *st) {
   else if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_plane_g2p2g<true, KM_DP>), g, b, 0, S(st), d);
   else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ELASTIC | KM_GK>), g, b, 0, S(st), d);
   else hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ALL>), g, b, 0, S(st), d);
+}
+void launch_plane_mb_claim(const DevPtrs &d, void *st) {
+  const int g = std::max(8, std::min(4096, blocks((long long)d.ntiles * PC_N, WG)));
+  hipLaunchKernelGGL(k_plane_mb_claim, dim3(g), dim3(WG), 0, S(st), d);
+}
+void launch_plane_mb_p2g(const DevPtrs &d, void *st) {
+  if (d.np > 0) hipLaunchKernelGGL(k_plane_mb_p2g, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
+}
+void launch_plane_mb_grid(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_plane_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+}
+void launch_plane_mb_update(const DevPtrs &d, void *st) {
+  hipLaunchKernelGGL(k_plane_mb_update, dim3(d.persist_wgs), dim3(WG), 0, S(st), d);
 }
 void launch_plane_grid(const DevPtrs &d, void *st) {
   const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));

@@ -68,6 +68,10 @@ struct Sim {
   // false: the gradient moves with its particle at a sort (a test knob modelling the GPU
   // path); true: the reference's permutation (see sort_by_cell)
   bool tgrad_as_reference = true;
+  // true: the GPU path's binning (a test knob, not the reference): every sort bins by the
+  // stencil base cell int((x - psep/2)/h) of the current position, and a cell's particles
+  // are ordered by object id, which is the order in which the GPU path claims body slots
+  bool gpu_binning = false;
   std::vector<int> cell, mat, obj, pbcType, pid;
   std::vector<char> hold;
   std::vector<double> pbcVal;  // 4 per particle
@@ -137,6 +141,12 @@ struct ByCell {
   const std::vector<int> *key;
   bool operator()(int i, int j) const { return (*key)[i] < (*key)[j]; }
 };
+struct ByCellObject {
+  const std::vector<int> *key, *obj;
+  bool operator()(int i, int j) const {
+    return (*key)[i] < (*key)[j] || ((*key)[i] == (*key)[j] && (*obj)[i] < (*obj)[j]);
+  }
+};
 
 template <class T>
 void permute(std::vector<T> &v, const std::vector<int> &ord, int w) {
@@ -153,9 +163,18 @@ void sort_by_cell(Sim &s) {
   for (int c = 0; c < ncell; c++) s.cellCount[c] = 0;
   s.order.resize(s.np);
   for (int i = 0; i < s.np; i++) s.order[i] = i;
+  if (s.gpu_binning) {
+    const int Cx = s.Nx - 1, Cy = s.Ny - 1;
+    for (int i = 0; i < s.np; i++) {
+      int b[3] = {0, 0, 0};
+      for (int a = 0; a < (s.d3 ? 3 : 2); a++) b[a] = (int)((s.x[3 * i + a] - (s.psep / 2) - s.npos[a]) / s.h);
+      s.cell[i] = b[0] + b[1] * Cx + b[2] * Cx * Cy;
+    }
+  }
   for (int i = 0; i < s.np; i++)
     if (s.cell[i] >= 0) s.cellCount[s.cell[i]]++;
-  std::sort(s.order.begin(), s.order.end(), ByCell{&s.cell});
+  if (s.gpu_binning) std::sort(s.order.begin(), s.order.end(), ByCellObject{&s.cell, &s.obj});
+  else std::sort(s.order.begin(), s.order.end(), ByCell{&s.cell});
   const auto &o = s.order;
   permute(s.x, o, 3); permute(s.x0, o, 3); permute(s.u, o, 3); permute(s.vel, o, 3);
   permute(s.gvel, o, 3); permute(s.sig, o, 6); permute(s.eps, o, 6); permute(s.epsp, o, 6);
@@ -906,7 +925,7 @@ void update_particle(Sim &s, int mp) {
 // One step — Solver::iterate_CPU, Solver.cpp:86-292. Grid BC phases 1 and 3 write the
 // same values as phase 2 (Grid.cpp:862-1094 ignores `phase`), so only phase 2 is run.
 void iterate(Sim &s, double t, int step) {
-  if ((step % s.sort_every == 0) || (step == 1)) sort_by_cell(s);
+  if (s.gpu_binning || (step % s.sort_every == 0) || (step == 1)) sort_by_cell(s);
   const int N = s.numNodes, P = s.np;
   // The reference's chunk N/(omp_get_num_threads()*16) (Solver.cpp:120-249) is evaluated
   // inside the team, so it is 0 (libgomp then never advances a dynamic loop: the step
@@ -1073,6 +1092,16 @@ int32_t cref_set_node_bc(void *h, int32_t n, const int32_t *type, const double *
 double cref_dt(void *h) { return ((Sim *)h)->dt; }
 void cref_set_sort_every(void *h, int32_t k) { ((Sim *)h)->sort_every = k > 0 ? k : 10; }
 void cref_set_tgrad_as_reference(void *h, int32_t on) { ((Sim *)h)->tgrad_as_reference = on != 0; }
+void cref_set_gpu_binning(void *h, int32_t on) { ((Sim *)h)->gpu_binning = on != 0; }
+// Body-slot claims of every node after the last P2G (Extrapolation.cpp:191-208): objects of
+// slot 1 and 2 (-1: none) and their rigid flags.
+void cref_get_claims(void *h, int32_t *body2, uint8_t *rigid2) {
+  const Sim *s = (const Sim *)h;
+  for (size_t i = 0; i < s->body.size(); i++) {
+    if (body2) body2[i] = s->body[i];
+    if (rigid2) rigid2[i] = (uint8_t)s->rigid[i];
+  }
+}
 void cref_set_dt(void *h, double dt) { ((Sim *)h)->dt = dt; }
 int32_t cref_num_nodes(void *h) { return ((Sim *)h)->numNodes; }
 

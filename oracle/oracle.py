@@ -84,11 +84,13 @@ class CpuSim:
     """One oracle simulation of a `graphyt_amd.workloads.Scene`."""
 
     def __init__(self, scene, kind: str = "restatement", sort_every: int | None = None,
-                 tgrad_moves: bool = False):
+                 tgrad_moves: bool = False, gpu_binning: bool = False):
         """Test knobs of the restatement (not the reference): sort_every re-bins every k
         steps instead of the reference's cadence (steps divisible by 10, and step 1), 1
         every step; tgrad_moves carries each particle's temperature gradient with it at a
-        sort, where the reference scrambles it (cpu_ref.cpp sort_by_cell)."""
+        sort, where the reference scrambles it (cpu_ref.cpp sort_by_cell); gpu_binning
+        re-bins every step by stencil-base cell with a cell's particles in object order,
+        as the GPU path bins and claims body slots."""
         self.kind = kind
         if kind == "reference":
             # The reference's dynamic OpenMP chunk, P/(16*threads), is evaluated inside the
@@ -114,9 +116,12 @@ class CpuSim:
             rc = self._f("set_node_bc")(self.h, len(t), t.ctypes.data_as(C.POINTER(C.c_int32)), _dp(v))
             if rc != 0:
                 raise RuntimeError("oracle set_node_bc failed")
-        if sort_every is not None or tgrad_moves:
+        if sort_every is not None or tgrad_moves or gpu_binning:
             if kind != "restatement":
-                raise ValueError("sort_every and tgrad_moves are knobs of the restatement only")
+                raise ValueError("sort_every, tgrad_moves and gpu_binning are knobs of the restatement only")
+        if gpu_binning:
+            self.lib.cref_set_gpu_binning.argtypes = [C.c_void_p, C.c_int32]
+            self.lib.cref_set_gpu_binning(self.h, 1)
         if sort_every is not None:
             self.lib.cref_set_sort_every.argtypes = [C.c_void_p, C.c_int32]
             self.lib.cref_set_sort_every(self.h, int(sort_every))

@@ -27,7 +27,11 @@ and the plane (2D, Parameters is3D = False) path, per unit depth:
   plane_grady_kipp      the same square with Grady-Kipp damage (closed-form 2D branch),
                         partly and fully damaged particles
   plane_brazilian       Drucker-Prager disk with damage between two rigid platens
-                        (brazillian2D.py), three objects
+                        (brazillian2D.py), three objects, positions jittered by 10 % of
+                        psep (on the exact lattice the reference's contact is
+                        ill-conditioned: particles on a node's support edge)
+  plane_rigid_platen    elastic square pressed by a rigid platen (two objects), jittered
+  plane_two_blocks      two deformable halves of a square in contact, sliding, friction 0.3
   plane_thermal         a centred elastic square with a temperature gradient (conductivity
                         0.01: the reference's plane heat update is unstable at copper's 391)
 
@@ -44,11 +48,18 @@ from graphyt_amd.abi import BC, material
 NAMES = ["elastic", "newtonian", "turbulent", "perfect_plastic", "dp_cohesive", "dp_unguarded", "grady_kipp",
          "bcs", "rigid_platen", "two_blocks_mu", "thermal"]
 PLANE_NAMES = ["plane_dam_break", "plane_elastic", "plane_perfect_plastic", "plane_turbulent",
-               "plane_grady_kipp", "plane_brazilian", "plane_thermal"]
+               "plane_grady_kipp", "plane_brazilian", "plane_rigid_platen", "plane_two_blocks", "plane_thermal"]
 ALL_NAMES = NAMES + PLANE_NAMES
 # scenes whose particles cross the periodic wrap: the reference's sort cadence (a wrapped
 # particle stays listed at the far side until the next sort) shows in their results
 WRAPPING = ["plane_elastic", "plane_perfect_plastic", "plane_grady_kipp"]
+# contact scenes where particles cross cells between the reference's sorts: a node claims
+# its body slots in the gather order over the cells of the last sort
+STALE_CLAIMS = ["plane_rigid_platen"]
+# contact scenes with cells holding particles of two objects: the body slot a node gives
+# each object follows the order particles are met in (std::sort's order of equal keys in
+# the reference, object id on the GPU), and body 2's normal is minus body 1's
+CLAIM_ORDER = ["plane_brazilian", "plane_rigid_platen"]
 # scenes with heat transfer: at a sort the reference scrambles the temperature gradient
 # (cpu_ref.cpp sort_by_cell), the GPU path carries it with its particle
 THERMAL = ["thermal", "plane_thermal"]
@@ -172,7 +183,31 @@ def build(name: str):
                      crack_speed=0.5, damage_m=9.0, damage_k=1e40)
         return W.square_2d(steps=60, mat=m), 60, None
     if name == "plane_brazilian":
-        return W.brazilian_disk_2d(h=0.004, v=20.0, steps=300), 300, None
+        return W.brazilian_disk_2d(h=0.004, v=2.0, steps=300, jitter=0.1), 300, None
+    if name == "plane_rigid_platen":
+        m = material(model=1, bulk_mod=1e6, shear_mod=5e5, density=1000.0)
+        sc = W.square_2d(steps=80, mat=m, centered=True, vx=0.0)
+        sc.gravity = (0.0, 0.0, 0.0)
+        pos = sc.particles["pos"].copy()
+        pos[:, :2] += np.random.default_rng(W.SEED).uniform(-0.05, 0.05, size=(len(pos), 2)) * sc.psep
+        top = pos[:, 1] > np.quantile(pos[:, 1], 0.85)
+        vel = np.zeros_like(pos)
+        vel[top, 1] = -0.3
+        sc.mats = [m, material(is_rigid=True)]
+        sc.particles.arrays.update(pos=pos, vel=vel, material=np.where(top, 1, 0).astype(np.int32),
+                                   obj_id=np.where(top, 2, 1).astype(np.int32))
+        return sc, 80, None
+    if name == "plane_two_blocks":
+        sc = W.square_2d(steps=60, centered=True, vx=0.0)
+        sc.gravity = (0.0, 0.0, 0.0)
+        sc.mu = 0.3
+        pos = sc.particles["pos"]
+        left = pos[:, 0] < np.median(pos[:, 0])
+        vel = np.zeros_like(pos)
+        vel[left, 0], vel[~left, 0] = 0.5, -0.5
+        vel[left, 1], vel[~left, 1] = 0.2, -0.2
+        sc.particles.arrays.update(obj_id=np.where(left, 1, 2).astype(np.int32), vel=vel)
+        return sc, 60, None
     if name in ("thermal", "plane_thermal"):
         if name == "thermal":
             sc = _cube(40)

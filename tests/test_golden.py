@@ -94,7 +94,9 @@ def test_binning_every_step_matches_the_reference_cadence(oracle_mod, name):
     (Extrapolation.cpp:52-88). The two agree wherever no particle leaves the reach of its
     listed cell between sorts: every fixture scene except the periodic crossing, where a
     wrapped particle stays listed at the far side and the reference drops its P2G
-    contributions until the next sort (scenes.WRAPPING)."""
+    contributions until the next sort (scenes.WRAPPING), and in contact scenes whose
+    particles cross cells, where node claims follow the stale cell lists
+    (scenes.STALE_CLAIMS)."""
     fx = load(name)
     sc, steps, t_of = scenes.build(name)
     sim = oracle_mod.CpuSim(sc, "restatement", sort_every=1)
@@ -102,12 +104,16 @@ def test_binning_every_step_matches_the_reference_cadence(oracle_mod, name):
     out = sim.get(["pos", "vel", "stress"])
     sim.close()
     r = {k: float(np.linalg.norm(out[k] - fx[k]) / np.linalg.norm(fx[k])) for k in out}
-    if name in scenes.WRAPPING:
-        assert r["stress"] > 1e-3  # the wrapped front loses its grid contributions there
+    if name in scenes.WRAPPING or name in scenes.STALE_CLAIMS:
+        # the wrapped front loses its grid contributions; stale claims swap body slots
+        assert r["stress"] > 1e-3
     elif name == "dp_unguarded":
         assert np.array_equal(np.isnan(out["stress"]), np.isnan(fx["stress"]))
     else:
-        tol = 1e-10 if name in ("grady_kipp",) else 1e-12
+        # the jittered Brazilian disk: a node claims its body slots in the gather order over
+        # the cells of the last sort (Extrapolation.cpp:191-208), and a few particles that
+        # cross cells between sorts change a claim
+        tol = {"grady_kipp": 1e-10, "plane_brazilian": 1e-4}.get(name, 1e-12)
         assert max(r.values()) < tol, r
 
 
@@ -127,3 +133,27 @@ def test_thermal_fixture_and_the_gradient_scramble(oracle_mod, name):
     sim.close()
     assert np.array_equal(out["pos"], fx["pos"]) and np.array_equal(out["stress"], fx["stress"])
     assert not np.array_equal(out["temperature"], fx["temperature"])
+
+
+@pytest.mark.parametrize("name", scenes.ALL_NAMES)
+def test_gpu_binning_oracle_against_the_fixtures(oracle_mod, name):
+    """The oracle binning as the GPU path does (every step, by stencil-base cell, a cell's
+    particles in object-id order: the order in which body slots are claimed) against the
+    reference's fixtures: equal up to FP64 summation order everywhere except the periodic
+    crossing (scenes.WRAPPING) and the contact scenes whose cells hold two objects
+    (scenes.CLAIM_ORDER), where the body-slot order differs at a few nodes."""
+    fx = load(name)
+    sc, steps, t_of = scenes.build(name)
+    sim = oracle_mod.CpuSim(sc, "restatement", gpu_binning=True)
+    run(sim, steps, t_of)
+    out = sim.get(["pos", "vel", "stress"])
+    sim.close()
+    r = {k: float(np.linalg.norm(out[k] - fx[k]) / np.linalg.norm(fx[k])) for k in out}
+    if name in scenes.WRAPPING:
+        assert r["stress"] > 1e-3
+    elif name in scenes.CLAIM_ORDER:
+        assert r["pos"] < 1e-5 and r["stress"] > 1e-3, r
+    elif name == "dp_unguarded":  # which particles go NaN follows FP64 summation order
+        assert r["pos"] < 1e-12
+    else:
+        assert max(r.values()) < 1e-12, r
