@@ -489,16 +489,16 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   const long long cap = d.cap;
   int q[3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) q[a] = (int)A[(F_QX + a) * cap + slot];
-  const uint32_t tag = A[F_TAG * cap + slot];
+  for (int a = 0; a < 3; a++) q[a] = (int)ld_stream(&A[(F_QX + a) * cap + slot]);
+  const uint32_t tag = ld_stream(&A[F_TAG * cap + slot]);
   const MatDev &M = s_mat[tag_mat(tag)];
   float v[3], s[6];
 #pragma unroll
-  for (int a = 0; a < 3; a++) v[a] = f32(A[(F_VX + a) * cap + slot]);
+  for (int a = 0; a < 3; a++) v[a] = f32(ld_stream(&A[(F_VX + a) * cap + slot]));
 #pragma unroll
-  for (int a = 0; a < 6; a++) s[a] = f32(A[(F_S0 + a) * cap + slot]);
-  float rho = f32(A[F_RHO * cap + slot]);
-  const float mass = f32(A[F_MASS * cap + slot]);
+  for (int a = 0; a < 6; a++) s[a] = f32(ld_stream(&A[(F_S0 + a) * cap + slot]));
+  float rho = f32(ld_stream(&A[F_RHO * cap + slot]));
+  const float mass = f32(ld_stream(&A[F_MASS * cap + slot]));
   int knew = -1;
 
   if (UPDATE) {
@@ -575,7 +575,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
 
     const int bct = (MASK & KM_PBC) ? tag_bct(tag) : 0;
     float bv[4] = {0, 0, 0, 0};
-    const int pid = (int)A[F_PID * cap + slot];
+    const int pid = (int)ld_stream(&A[F_PID * cap + slot]);
     if (MASK & KM_PBC) {
       if (bct) {
 #pragma unroll
@@ -595,7 +595,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     }
     float e[6];
 #pragma unroll
-    for (int a = 0; a < 6; a++) e[a] = f32(A[(F_E0 + a) * cap + slot]) + er[a] * dt;
+    for (int a = 0; a < 6; a++) e[a] = f32(ld_stream(&A[(F_E0 + a) * cap + slot])) + er[a] * dt;
 #pragma unroll
     for (int a = 0; a < 6; a++) B[(F_E0 + a) * cap + out] = u32(e[a]);
     if (!M.rigid) {
@@ -604,8 +604,8 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       if ((MASK & (KM_PERFECT | KM_DP)) && M.carry_plastic) {
         float p[6];
 #pragma unroll
-        for (int a = 0; a < 6; a++) p[a] = f32(A[(F_P0 + a) * cap + slot]);
-        float epb = f32(A[F_EPB * cap + slot]);
+        for (int a = 0; a < 6; a++) p[a] = f32(ld_stream(&A[(F_P0 + a) * cap + slot]));
+        float epb = f32(ld_stream(&A[F_EPB * cap + slot]));
         if ((MASK & KM_PERFECT) && M.model == 3) cm_perfect(M, er, s, p, epb, dt);
         if ((MASK & KM_DP) && M.model == 4) cm_drucker_prager(M, er, s, p, epb, dt);
 #pragma unroll
@@ -614,7 +614,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       }
       if (MASK & (KM_GK | KM_TURB)) {
         if (M.carry_damage) {
-          float dmg = f32(A[F_DMG * cap + slot]), smax = f32(A[F_SMAX * cap + slot]);
+          float dmg = f32(ld_stream(&A[F_DMG * cap + slot])), smax = f32(ld_stream(&A[F_SMAX * cap + slot]));
           if ((MASK & KM_TURB) && M.model == 5) cm_turbulent(M, er, e, v, mass, dmg, s, dt);
           if ((MASK & KM_GK) && M.gk) gk_damage(M, s, dmg, smax, mass, rho, dt);
           B[F_DMG * cap + out] = u32(dmg);
@@ -625,23 +625,23 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       // rigid particles keep their stress, plastic and damage fields
       if (M.carry_plastic) {
 #pragma unroll
-        for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = A[(F_P0 + a) * cap + slot];
-        B[F_EPB * cap + out] = A[F_EPB * cap + slot];
+        for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = ld_stream(&A[(F_P0 + a) * cap + slot]);
+        B[F_EPB * cap + out] = ld_stream(&A[F_EPB * cap + slot]);
       }
       if (M.carry_damage) {
-        B[F_DMG * cap + out] = A[F_DMG * cap + slot];
-        B[F_SMAX * cap + out] = A[F_SMAX * cap + slot];
+        B[F_DMG * cap + out] = ld_stream(&A[F_DMG * cap + slot]);
+        B[F_SMAX * cap + out] = ld_stream(&A[F_SMAX * cap + slot]);
       }
     }
     if (!(MASK & (KM_PERFECT | KM_DP)) && M.carry_plastic && !M.rigid) {
       // plastic fields uploaded for a model that never changes them
 #pragma unroll
-      for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = A[(F_P0 + a) * cap + slot];
-      B[F_EPB * cap + out] = A[F_EPB * cap + slot];
+      for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = ld_stream(&A[(F_P0 + a) * cap + slot]);
+      B[F_EPB * cap + out] = ld_stream(&A[F_EPB * cap + slot]);
     }
     if (!(MASK & (KM_GK | KM_TURB)) && M.carry_damage && !M.rigid) {
-      B[F_DMG * cap + out] = A[F_DMG * cap + slot];
-      B[F_SMAX * cap + out] = A[F_SMAX * cap + slot];
+      B[F_DMG * cap + out] = ld_stream(&A[F_DMG * cap + slot]);
+      B[F_SMAX * cap + out] = ld_stream(&A[F_SMAX * cap + slot]);
     }
     // periodic wrap with period L-2h (TimeIntegration.cpp:80-108)
 #pragma unroll
@@ -666,7 +666,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     B[F_PID * cap + out] = (uint32_t)pid;
     if (d.thermal) {  // temperature and gradient travel with the particle (gy_thermal.hip)
 #pragma unroll
-      for (int a = 0; a < 4; a++) B[(F_T + a) * cap + out] = A[(F_T + a) * cap + slot];
+      for (int a = 0; a < 4; a++) B[(F_T + a) * cap + out] = ld_stream(&A[(F_T + a) * cap + slot]);
     }
 
     // ---- new binning key ----
