@@ -312,11 +312,18 @@ __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc
 // Grid update of one flagged canonical brick (one wave per 4x4x4 brick): gather the P2G
 // partials of every actual node of the periodic class, then solve.
 __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
-  const InstDesc &I = d.inst[d.brick_inst[gb]];
+  // one instance: no brick -> instance lookup in the load chain
+  const InstDesc &I = d.inst[d.ninst == 1 ? 0 : d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
   const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
   if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
+    // the canonical node's direct-scatter sums and BC, loaded before the gather's chain
+    const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
+    float4 *ap = reinterpret_cast<float4 *>(d.acc + ci * NODE_W);
+    const float4 x0 = ap[0], x1 = ap[1];
+    const int bt0 = d.bc_type[ci];
+    const float bv0 = d.bc_val[2 * ci], bv1 = d.bc_val[2 * ci + 1];
     // P2G sums over the periodic class (Grid.cpp:320-602: node i and node N-3+i share sums)
     float a7[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -333,9 +340,6 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
       }
     }
     // particles scattered straight to the grid (outside their tile's region)
-    const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
-    float4 *ap = reinterpret_cast<float4 *>(d.acc + ci * NODE_W);
-    const float4 x0 = ap[0], x1 = ap[1];
     if (x0.x != 0.f || x0.y != 0.f || x0.z != 0.f || x0.w != 0.f || x1.x != 0.f || x1.y != 0.f || x1.z != 0.f) {
       a7[0] += x0.x; a7[1] += x0.y; a7[2] += x0.z; a7[3] += x0.w;
       a7[4] += x1.x; a7[5] += x1.y; a7[6] += x1.z;
@@ -370,9 +374,10 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
           const int n0 = c0 + mx * I.cper[0];
           const long long ni = I.node_base + n0 + (long long)I.n[0] * (n1 + (long long)I.n[1] * n2);
           float vv[3] = {v[0], v[1], v[2]}, aa[3] = {acc[0], acc[1], acc[2]};
-          const int t = d.bc_type[ni];
+          const bool canon = (ni == ci);
+          const int t = canon ? bt0 : d.bc_type[ni];
           if (t == 1 || t == 2 || t == 11) {
-            const float b0 = d.bc_val[2 * ni], b1 = d.bc_val[2 * ni + 1];
+            const float b0 = canon ? bv0 : d.bc_val[2 * ni], b1 = canon ? bv1 : d.bc_val[2 * ni + 1];
             if (t == 1) { vv[0] = b0; aa[0] = 0.f; }
             else if (t == 2) { vv[1] = b0; aa[1] = 0.f; }
             else { vv[0] = b0; vv[1] = b1; aa[0] = 0.f; aa[1] = 0.f; }

@@ -204,6 +204,24 @@ struct MaterialPoints {
     for (int i = 0; i < n; i++) add((int)size(), p[3 * i], p[3 * i + 1], p[3 * i + 2]);
     numParticles = (int)size();
   }
+  // addPyckIntField(field) (MaterialPoints.cpp:32-53): a field named "material" or
+  // "objectID" sets that per-particle array from field.data (pyck's IntField, or
+  // graphyt.IntField); other names are ignored. addPyckDoubleField is a no-op there too.
+  void addPyckIntField(py::object field) {
+    const std::string name = py::str(field.attr("name"));
+    auto data = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(field.attr("data"));
+    const size_t n = size();
+    if ((size_t)data.size() < n) throw std::invalid_argument("field holds fewer values than particles");
+    const int32_t *v = data.data();
+    if (name == "material") {
+      touch();
+      for (size_t i = 0; i < n; i++) material[i] = v[i];
+    } else if (name == "objectID") {
+      touch();
+      for (size_t i = 0; i < n; i++) objID[i] = v[i];
+    }
+  }
+  void addPyckDoubleField(py::object) {}
   size_t size() const { return position.size() / 3; }
 
   void pull();     // device -> host when a step ran since the last pull
@@ -661,6 +679,33 @@ PYBIND11_MODULE(graphyt, m) {
   using namespace gyhost;
   m.doc() = "graphyt surface (Nodes, MaterialPoints, MaterialModels, Parameters, Solver) over the MI355X step";
 
+  // IntField(name, n, dim) (intField.h): a named integer field with n*dim values in
+  // `data` (a numpy view), for addPyckIntField where pyck is not installed
+  struct IntField {
+    std::string name;
+    long n = 0;
+    int dim = 1;
+    std::vector<int32_t> data;
+  };
+  py::class_<IntField>(m, "IntField")
+      .def(py::init([](const std::string &name, long n, int dim) {
+             if (n < 0 || dim < 1) throw std::invalid_argument("IntField(name, n >= 0, dim >= 1)");
+             IntField f;
+             f.name = name;
+             f.n = n;
+             f.dim = dim;
+             f.data.assign((size_t)n * dim, 0);
+             return f;
+           }),
+           py::arg("name"), py::arg("n"), py::arg("dim") = 1)
+      .def_readonly("name", &IntField::name)
+      .def_readonly("n", &IntField::n)
+      .def_readonly("dim", &IntField::dim)
+      .def_property_readonly("data", [](py::object self) {
+        auto &f = self.cast<IntField &>();
+        return py::array_t<int32_t>({(py::ssize_t)f.data.size()}, {(py::ssize_t)sizeof(int32_t)}, f.data.data(), self);
+      });
+
   // BCTypes: graphyt.BCTypes.grid_vx etc. (integers, the reference's codes)
   {
     py::class_<BCTypes> bc(m, "BCTypes");
@@ -758,6 +803,8 @@ PYBIND11_MODULE(graphyt, m) {
       .def("_sync", &MaterialPoints::pull)  // particle state of the last step into the host arrays
       .def("add", &MaterialPoints::add, py::arg("i"), py::arg("x"), py::arg("y"), py::arg("z"))
       .def("addPyckGeom", &MaterialPoints::addPyckGeom, py::arg("pos"), py::arg("numParticle"))
+      .def("addPyckIntField", &MaterialPoints::addPyckIntField, py::arg("intField"))
+      .def("addPyckDoubleField", &MaterialPoints::addPyckDoubleField, py::arg("doubleField"))
       .def("getPosX", &MaterialPoints::getPosX)
       .def("getPosY", &MaterialPoints::getPosY)
       .def("getPosZ", &MaterialPoints::getPosZ)

@@ -23,3 +23,23 @@ def test_plane_grid_and_initialize():
     mass, vol = mp.getMassArr(), mp.getVolumeArr()
     assert mass[0] == pytest.approx(2600.0 * 1e-4) and vol[0] == pytest.approx(1e-4)
     assert mass[3] == pytest.approx(1000.0 * 1e-4)  # rigid keeps rho * V in the plane path
+
+
+def test_pyck_int_fields_set_material_and_object():
+    """addPyckIntField (MaterialPoints.cpp:32-53): "material" and "objectID" fields set the
+    particle arrays, other names are ignored; addPyckDoubleField does nothing."""
+    nodes = graphyt.Nodes([0.2, 0.2, 0.2], 0.02)
+    mp = graphyt.MaterialPoints(nodes)
+    mp.psep = 0.01
+    mp.addPyckGeom([0.05, 0.05, 0.05, 0.06, 0.05, 0.05, 0.07, 0.05, 0.05], 3)
+    mat = graphyt.IntField("material", 3, 1)
+    mat.data[:] = [0, 1, 1]
+    obj = graphyt.IntField("objectID", 3)
+    obj.data[:] = [4, 4, 7]
+    other = graphyt.IntField("color", 3)
+    other.data[:] = 9
+    for f in (mat, obj, other):
+        mp.addPyckIntField(f)
+    mp.addPyckDoubleField(None)
+    assert list(mp.getMatArr()) == [0, 1, 1]
+    assert [mp.getObjID(i) for i in range(3)] == [4, 4, 7]
