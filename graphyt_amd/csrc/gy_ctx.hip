@@ -660,6 +660,7 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     if ((rc = dalloc(c, &d.claim, (size_t)c->nnodes))) return rc;
     if ((rc = dalloc(c, &d.velb, (size_t)c->nnodes * MB_VELB_W))) return rc;
     if ((rc = dalloc(c, &d.cellobj, (size_t)d.nkeys))) return rc;
+    if ((rc = dalloc(c, &d.twotile, (size_t)d.ntiles))) return rc;
   }
   HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(d.pbc_val, pbc.data(), pbc.size() * 4, hipMemcpyHostToDevice, c->stream));

@@ -191,6 +191,7 @@ struct DevPtrs {
   int4 *claim;           // per node: object of body 1, object of body 2, rigid flags
   float *velb;           // MB_VELB_W per node: v1, a1, v2, a2 after the contact solve
   int4 *cellobj;         // per key: first and second object in the cell, rigid flags
+  int *twotile;          // per tile: a node of its core region has two claimed bodies
   // heat transfer (any uploaded temperature nonzero); see gy_thermal.hip
   int thermal;
   float *therm;          // 2 per node: heat content sum(N m cp T), conduction rate

@@ -1012,6 +1012,94 @@ __global__ __launch_bounds__(WG) void k_mb_claim(DevPtrs d) {
   }
 }
 
+// The same claims with one workgroup per active tile: the objects of the 9 x 9 x 9 cells
+// the windows of the tile's 6 x 6 x 6 core nodes cover, [o-2, o+7), are staged in LDS (an
+// empty cell holds no object), then each core node walks its 64 window cells from LDS in
+// the reference's order. One pass: a slot is claimed at the first cell holding its object,
+// so the rigid flag of a slot only needs the cells from there on. Nodes whose window
+// leaves the grid (where linear cell ids wrap to the next row) take the global walk.
+constexpr int CLAIM_R = CORE_R + 3;  // staged cell edge: windows g-2..g+1 of nodes o..o+CORE_R-1
+__global__ __launch_bounds__(WG) void k_mb_claim_tiles(DevPtrs d) {
+  __shared__ int4 s_co[CLAIM_R * CLAIM_R * CLAIM_R];
+  const int nact = d.ctrl->active_tiles;
+  for (int ta = blockIdx.x; ta < nact; ta += gridDim.x) {
+    const int t = d.active[ta];
+    const InstDesc &I = d.inst[d.tile_inst[t]];
+    const int lt = t - I.tile_base;
+    const int o0 = (lt % I.nt[0]) * TILE, o1 = ((lt / I.nt[0]) % I.nt[1]) * TILE, o2 = (lt / (I.nt[0] * I.nt[1])) * TILE;
+    for (int n = threadIdx.x; n < CLAIM_R * CLAIM_R * CLAIM_R; n += WG) {
+      const int c[3] = {o0 - 2 + n % CLAIM_R, o1 - 2 + (n / CLAIM_R) % CLAIM_R, o2 - 2 + n / (CLAIM_R * CLAIM_R)};
+      int4 co = make_int4(-1, -1, 0, 0);
+      if (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] <= I.n[0] - 3 && c[1] <= I.n[1] - 3 && c[2] <= I.n[2] - 3) {
+        const int k = key_of(I, c);
+        if (d.offsets[k + 1] != d.offsets[k]) co = d.cellobj[k];
+      }
+      s_co[n] = co;
+    }
+    __syncthreads();
+    int two = 0;
+    if (threadIdx.x < CORE_NODES) {
+      const int ln = threadIdx.x;
+      const int g0 = o0 + ln % CORE_R, g1 = o1 + (ln / CORE_R) % CORE_R, g2 = o2 + ln / (CORE_R * CORE_R);
+      const int Cx = I.n[0] - 1, Cy = I.n[1] - 1, Cz = I.n[2] - 1;
+      if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
+        int b1 = -1, b2 = -1, r1 = 0, r2 = 0;
+        if (g0 >= 2 && g1 >= 2 && g2 >= 2 && g0 + 1 < Cx && g1 + 1 < Cy && g2 + 1 < Cz) {
+          // window inside the grid: cell (g + d) for d in [-2, 1]^3, in (k, j, i) order
+          for (int kz = -2; kz < 2; kz++)
+            for (int jy = -2; jy < 2; jy++)
+              for (int ix = -2; ix < 2; ix++) {
+                const int4 co = s_co[(g0 + ix - (o0 - 2)) + CLAIM_R * ((g1 + jy - (o1 - 2)) + CLAIM_R * (g2 + kz - (o2 - 2)))];
+                const int ob[2] = {co.x, co.y}, rg[2] = {co.z, co.w};
+                for (int q = 0; q < 2; q++) {
+                  if (ob[q] < 0) continue;
+                  if (b1 < 0) b1 = ob[q];
+                  else if (b2 < 0 && ob[q] != b1) b2 = ob[q];
+                  if (ob[q] == b1) r1 |= rg[q];
+                  else if (ob[q] == b2) r2 |= rg[q];
+                }
+              }
+        } else {
+          const long long ncell = (long long)Cx * Cy * Cz;
+          for (int pass = 0; pass < 2; pass++)
+            for (int kz = -2; kz < 2; kz++)
+              for (int jy = -2; jy < 2; jy++)
+                for (int ix = -2; ix < 2; ix++) {
+                  const long long lin = (g0 + ix) + (long long)(g1 + jy) * Cx + (long long)(g2 + kz) * Cx * Cy;
+                  if (lin < 0 || lin >= ncell) continue;
+                  const int c[3] = {(int)(lin % Cx), (int)((lin / Cx) % Cy), (int)(lin / ((long long)Cx * Cy))};
+                  if (c[0] > I.n[0] - 3 || c[1] > I.n[1] - 3 || c[2] > I.n[2] - 3) continue;
+                  const int k = key_of(I, c);
+                  if (d.offsets[k + 1] == d.offsets[k]) continue;
+                  const int4 co = d.cellobj[k];
+                  if (pass == 0) {
+                    const int ob[2] = {co.x, co.y};
+                    for (int q = 0; q < 2; q++) {
+                      if (ob[q] < 0) continue;
+                      if (b1 < 0) b1 = ob[q];
+                      else if (b2 < 0 && ob[q] != b1) b2 = ob[q];
+                    }
+                  } else {
+                    r1 |= (co.x == b1 && co.z) || (co.y == b1 && co.w);
+                    r2 |= b2 >= 0 && ((co.x == b2 && co.z) || (co.y == b2 && co.w));
+                  }
+                }
+        }
+        d.claim[I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2)] = make_int4(b1, b2, r1, r2);
+        two = b2 >= 0;
+      }
+    }
+    // the tile's particles reach only its core nodes: without a two-body node there, its
+    // non-rigid particles have no body fields to scatter (k_mb_p2g skips them)
+    if (__syncthreads_or(two) ? 1 : 0) {
+      if (threadIdx.x == 0) d.twotile[t] = 1;
+    } else if (threadIdx.x == 0) {
+      d.twotile[t] = 0;
+    }
+    __syncthreads();
+  }
+}
+
 // Body-field P2G of one particle (slot order) at the nodes where two bodies are claimed:
 // body fields to the slot its object holds at the actual node, folded onto the canonical
 // node (rigid particles: unit mass, no gravity in their body's force,
@@ -1020,55 +1108,95 @@ __global__ __launch_bounds__(WG) void k_mb_claim(DevPtrs d) {
 // sum at every node. FP32 global atomics.
 __global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
-  if (s >= d.np) return;
+  const int lane = threadIdx.x & 63;
+  const bool live = s < d.np;
+  const int sc = live ? s : d.np - 1;  // dead lanes read a valid slot and contribute nothing
   const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
   const long long cap = d.cap;
-  const int ii = inst_of_slot(d, s);
+  const int ii = inst_of_slot(d, sc);
   const InstDesc &I = d.inst[ii];
-  const uint32_t tag = A[F_TAG * cap + s];
+  const uint32_t tag = A[F_TAG * cap + sc];
   const MatDev &M = d.mats[I.mat_base + tag_mat(tag)];
+  // the rigid-mass sums only remove the rigid share of m*g from the body forces
+  const bool rigid_mass = M.rigid && (I.gravity[0] != 0.0f || I.gravity[1] != 0.0f || I.gravity[2] != 0.0f);
+  const int key = (d.ctrl->parity ? d.key[1] : d.key[0])[sc];
+  const bool active = live && (rigid_mass || d.twotile[key / TILE3]);
+  if (__ballot(active) == 0ull) return;  // wave-uniform
   const int obj = tag_obj(tag);
   int q[3], b[3];
   bool bad = false;
   Axis W[3];
   for (int a = 0; a < 3; a++) {
-    q[a] = (int)A[(F_QX + a) * cap + s];
+    q[a] = (int)A[(F_QX + a) * cap + sc];
     b[a] = stencil_base(q[a], I.lpq, I.fbits, I.n[a] - 3, bad);
     W[a] = gimp3((float)(q[a] - (b[a] << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
   }
-  const float mass = f32(A[F_MASS * cap + s]), rho = f32(A[F_RHO * cap + s]);
+  const float mass = f32(A[F_MASS * cap + sc]), rho = f32(A[F_RHO * cap + sc]);
   const float mp = M.rigid ? 1.0f : mass;
   P2GTerms T;
   T.m = mp;
-  for (int a = 0; a < 3; a++) T.mv[a] = mp * f32(A[(F_VX + a) * cap + s]);
+  for (int a = 0; a < 3; a++) T.mv[a] = mp * f32(A[(F_VX + a) * cap + sc]);
   const float V = mp / rho;
-  for (int a = 0; a < 6; a++) T.vs[a] = -V * f32(A[(F_S0 + a) * cap + s]);
+  for (int a = 0; a < 6; a++) T.vs[a] = -V * f32(A[(F_S0 + a) * cap + sc]);
+  // Adjacent lanes with the same stencil base node, object and rigid-mass flag -- consecutive
+  // slots are mostly the particles of one cell -- share all 27 target nodes and body slots:
+  // their terms are summed over the run (segmented shuffle reduction) and the run's first
+  // lane issues the atomics, which cuts the contended global atomics by the particles per
+  // cell. Runs are numbered by the heads at or below a lane, so two runs of equal base in one
+  // wave stay apart.
+  const int base = (int)(I.node_base + b[0] + (long long)I.n[0] * (b[1] + (long long)I.n[1] * b[2]));
+  const int seg_base = active ? base : -1 - lane;
+  const int seg_obj = obj * 2 + (rigid_mass ? 1 : 0);
+  // Every shuffle runs with the whole wave active (a lane switched off reads as 0), so none
+  // sits behind a short-circuit test.
+  const int up_base = __shfl_up(seg_base, 1, 64), up_obj = __shfl_up(seg_obj, 1, 64);
+  const bool head = lane == 0 || up_base != seg_base || up_obj != seg_obj;
+  const unsigned long long heads = __ballot(head);
+  const int run = __popcll(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull)));
+  bool same[6];
+#pragma unroll
+  for (int r = 0, off = 1; r < 6; r++, off <<= 1) {
+    const int down_run = __shfl_down(run, off, 64);
+    same[r] = lane + off < 64 && down_run == run;
+  }
   for (int k = 0; k < 3; k++)
     for (int j = 0; j < 3; j++)
       for (int i = 0; i < 3; i++) {
         const float A_ = W[1].w[j] * W[2].w[k];
         const float w = W[0].w[i] * A_;
         const float gx = W[0].g[i] * A_, gy = W[0].w[i] * W[1].g[j] * W[2].w[k], gz = W[0].w[i] * W[1].w[j] * W[2].g[k];
-        if (w == 0.0f && gx == 0.0f && gy == 0.0f && gz == 0.0f) continue;
         const int g[3] = {b[0] + i, b[1] + j, b[2] + k};
         const long long gi = I.node_base + g[0] + (long long)I.n[0] * (g[1] + (long long)I.n[1] * g[2]);
         const int4 cl = d.claim[gi];
         const bool two = cl.y != -1;
-        if (!two && !M.rigid) continue;
         const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
-        if (sl < 0) continue;
+        const bool use = active && sl >= 0 && (two || rigid_mass);  // uniform over a run
+        float v[11];
+        node_terms(T, w, gx, gy, gz, v);
+        v[7] = gx * V;
+        v[8] = gy * V;
+        v[9] = gz * V;
+        v[10] = w * T.m;
+#pragma unroll
+        for (int f = 0; f < 11; f++) v[f] = use ? v[f] : 0.0f;
+#pragma unroll
+        for (int r = 0, off = 1; r < 6; r++, off <<= 1)
+#pragma unroll
+          for (int f = 0; f < 11; f++) {
+            const float o = __shfl_down(v[f], off, 64);
+            v[f] += same[r] ? o : 0.0f;
+          }
+        if (!(use && head)) continue;
         int c[3];
         const long long ci = canon_index(I, g, c);
         float *ab = d.acc2 + ci * MB_ACC2_W + sl * 8;
-        if (M.rigid) unsafeAtomicAdd(&ab[7], w * T.m);
+        if (rigid_mass) unsafeAtomicAdd(&ab[7], v[10]);
         if (!two) continue;
-        float t[7];
-        node_terms(T, w, gx, gy, gz, t);
-        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], t[f]);
+        for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], v[f]);
         float *nb = d.nrm + gi * 6 + sl * 3;
-        unsafeAtomicAdd(&nb[0], gx * V);
-        unsafeAtomicAdd(&nb[1], gy * V);
-        unsafeAtomicAdd(&nb[2], gz * V);
+        unsafeAtomicAdd(&nb[0], v[7]);
+        unsafeAtomicAdd(&nb[1], v[8]);
+        unsafeAtomicAdd(&nb[2], v[9]);
       }
 }
 
@@ -1312,9 +1440,8 @@ void launch_mb_cells(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_mb_cells, dim3(std::max(8, std::min(4096, blocks(d.ntiles, WG / TILE3)))), dim3(WG), 0, S(st), d);
 }
 void launch_mb_claim(const DevPtrs &d, void *st) {
-  const int g = std::max(8, std::min(4096, blocks((long long)d.ntiles * CORE_NODES, WG)));
   launch_mb_cells(d, st);
-  hipLaunchKernelGGL(k_mb_claim, dim3(g), dim3(WG), 0, S(st), d);
+  hipLaunchKernelGGL(k_mb_claim_tiles, dim3(std::max(8, std::min(8192, d.ntiles))), dim3(WG), 0, S(st), d);
 }
 void launch_mb_p2g(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_mb_p2g, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
