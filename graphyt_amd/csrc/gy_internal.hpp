@@ -229,7 +229,7 @@ void launch_stable_order(const DevPtrs &d, int *order, void *stream);
 // two-body contact step (Grid.cpp:9-308, Extrapolation.cpp:166-367): order of one step is
 // mb_grid, mb_update, scan_partial (+ advance), scan_final, place, mb_claim, p2g_only, mb_p2g
 void launch_mb_grid(const DevPtrs &d, void *stream);
-void launch_mb_update(const DevPtrs &d, void *stream);
+void launch_mb_g2p2g(const DevPtrs &d, void *stream);
 void launch_mb_claim(const DevPtrs &d, void *stream);   // cell objects + node body claims
 void launch_mb_p2g(const DevPtrs &d, void *stream);
 int scan_items_per_block();

@@ -482,7 +482,7 @@ struct RecLds {
 // (ay, az, -, -)}, two 16-byte LDS reads per node.
 // MB (two-body contact): G2P reads the velocity field of the body slot the particle's
 // object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots' objects;
-// Extrapolation.cpp:288-367), and no record is written (P2G runs after the re-binning).
+// Extrapolation.cpp:288-367). The record is written when s_rec is given.
 template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
                                                int i, int slot, const float4 (*s_vel)[2], const MatDev *s_mat,
@@ -683,7 +683,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
   }
 
-  if (MB) return knew;
+  if (s_rec == nullptr) return knew;
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
@@ -693,6 +693,31 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   w[2] = make_float4(nV * s[1], nV * s[2], nV * s[3], nV * s[4]);
   s_rec->t[RecLds::ti(j)] = nV * s[5];
   return knew;
+}
+
+// Contact G2P staging of node n of a tile's G2P region: both body slots' velocity and
+// acceleration and the objects that hold them; zero at nodes without mass, which G2P
+// skips (Extrapolation.cpp:290).
+__device__ __forceinline__ void stage_body_vel(const DevPtrs &d, const InstDesc &I, const int o[3], int n,
+                                               float4 (*s_vel)[2], float4 (*s_vel2)[2], int (*s_obj)[NODES_G2P]) {
+  const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
+  const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
+  float vq[MB_VELB_W];
+  for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
+  int4 cl = make_int4(-1, -1, 0, 0);
+  if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
+    const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
+    if (d.vel[ni * NODE_W + 6] != 0.0f) {
+      for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
+      cl = d.claim[ni];
+    }
+  }
+  s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+  s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
+  s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
+  s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
+  s_obj[0][n] = cl.x;
+  s_obj[1][n] = cl.y;
 }
 
 // Phase B: GIMP weights of one record's position (Extrapolation.cpp:556-709).
@@ -779,11 +804,20 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 // chunk the region is written as FP32 partial sums with plain stores and the tile's epoch
 // is stamped; the grid kernel gathers the partials of the tiles covering each node in a
 // fixed order.
+//
+// MB (two-body contact): phase A is the contact G2P (body velocity fields s_vel, s_vel2
+// and the slots' objects staged per node) and phase B the centre-of-mass P2G; the body
+// fields follow the re-binning and the claims (k_mb_p2g). Body 2's fields take LDS
+// beyond three workgroups per CU, so the contact variant runs two.
 
-template <bool UPDATE, int MASK>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_g2p2g(DevPtrs d) {
+template <bool UPDATE, int MASK, bool MB = false>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))) void k_g2p2g(DevPtrs d) {
   static_assert(WG == 4 * TILE3, "four waves per tile");
+  static_assert(!MB || UPDATE, "the contact variant moves particles");
+  constexpr int NB2 = MB ? NODES_G2P : 1;
   __shared__ float4 s_vel[NODES_G2P][2];
+  __shared__ float4 s_vel2[NB2][2];
+  __shared__ int s_obj[2][NB2];
   __shared__ __align__(16) double s_acc[7][ACC_N];
   __shared__ RecLds s_rec;
   __shared__ int s_off[TILE3 + 1];
@@ -815,7 +849,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
     if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
     if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
-    if (UPDATE) {
+    if constexpr (MB) {
+      for (int n = tid; n < NODES_G2P; n += WG) stage_body_vel(d, I, o, n, s_vel, s_vel2, s_obj);
+    } else if (UPDATE) {
       for (int n = tid; n < NODES_G2P; n += WG) {
         const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
         const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
@@ -870,7 +906,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const int col = 16 * (j - jq * P) + jq;
       // phase A
       int knew = -1;
-      if (i < c1) knew = particle_update<UPDATE, MASK>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col);
+      if (i < c1)
+        knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col,
+                                                 MB ? s_vel2 : nullptr,
+                                                 MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr);
       int hl = 0, hbase = 0;
       if (UPDATE) hbase = hist_issue(d.count, knew, hl);
       __syncthreads();
@@ -936,14 +975,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // (Extrapolation.cpp:191-208); P2G adds every particle to the centre-of-mass fields and
 // to its object's slot (Extrapolation.cpp:210-280); the nodal solve corrects approaching
 // bodies (Grid.cpp:9-245, 247-308); G2P reads the slot of the particle's object
-// (Extrapolation.cpp:288-367). Here the step is unfused: mb_grid, mb_update (G2P +
-// update + store + new key), advance, binning, mb_claim (cell objects, then node claims
-// from the new binning), the centre-of-mass P2G of the tile kernel (k_g2p2g without the
-// update), and mb_p2g for the body fields. A node with one claimed body has body fields
-// equal to its centre-of-mass fields (every particle its window sees is of that object),
-// so body fields and normals are scattered only to nodes with two claimed bodies; rigid
-// particles also scatter their mass to their body's slot everywhere (gravity acts on the
-// body fields of non-rigid mass only).
+// (Extrapolation.cpp:288-367). Here the step is: mb_grid, the contact variant of the
+// fused tile kernel (G2P from the body slots + update + store + new key, then the
+// centre-of-mass P2G of the new state into the tile partials), advance, binning, mb_claim
+// (cell objects, then node claims from the new binning) and mb_p2g for the body fields. A
+// node with one claimed body has body fields equal to its centre-of-mass fields (every
+// particle its window sees is of that object), so body fields and normals are scattered
+// only to nodes with two claimed bodies; rigid particles also scatter their mass to their
+// body's slot wherever gravity is nonzero (gravity acts on the body fields of non-rigid
+// mass only).
 
 // Objects present in each cell of the active tiles: the two smallest object ids and
 // whether each has a rigid particle there. A cell's particles are met in ascending object
@@ -1259,68 +1299,6 @@ __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
   if (lane == 0) d.brick_flag[gb] = 0;
 }
 
-// G2P + update + constitutive + store + new key on the body-slot velocity fields,
-// persistent over the active tiles (as k_g2p2g, without the P2G phase).
-template <int MASK>
-__global__ __launch_bounds__(WG) void k_mb_update(DevPtrs d) {
-  __shared__ float4 s_vel[NODES_G2P][2], s_vel2[NODES_G2P][2];
-  __shared__ int s_obj[2][NODES_G2P];
-  __shared__ MatDev s_mat[MAXMAT];
-  __shared__ int s_off[TILE3 + 1];
-  __shared__ int s_tile;
-  const int tid = threadIdx.x;
-  const int par = d.ctrl->parity;
-  const TileQueue queue(d);
-  for (;;) {
-    if (tid == 0) {
-      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
-    }
-    __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(s_tile);
-    if (t < 0) break;
-    const int k0 = t * TILE3;
-    const int ii = d.tile_inst[t];
-    const InstDesc I = d.inst[ii];
-    const int lt = t - I.tile_base;
-    const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
-    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
-    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
-    for (int n = tid; n < NODES_G2P; n += WG) {
-      const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
-      const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
-      float vq[MB_VELB_W];
-      for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
-      int4 cl = make_int4(-1, -1, 0, 0);
-      if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
-        const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
-        if (d.vel[ni * NODE_W + 6] != 0.0f) {  // G2P skips nodes without mass (Extrapolation.cpp:290)
-          for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
-          cl = d.claim[ni];
-        }
-      }
-      s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
-      s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
-      s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
-      s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
-      s_obj[0][n] = cl.x;
-      s_obj[1][n] = cl.y;
-    }
-    __syncthreads();
-    const double tl = d.t_last[ii];
-    const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
-    for (int c0 = p0; c0 < p1; c0 += WG) {
-      const int i = c0 + tid;
-      int knew = -1;
-      if (i < p1)
-        knew = particle_update<true, MASK, true>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, nullptr, 0, s_vel2,
-                                                 s_obj);
-      const int r = wave_hist_rank(d.count, knew);
-      if (knew >= 0) d.rank[i] = r;
-    }
-    __syncthreads();
-  }
-}
-
 // FP64 audit: per block sums of mass, momentum, kinetic energy, a stress-energy proxy and
 // non-finite count.
 constexpr int AUD = 8;
@@ -1433,8 +1411,15 @@ void launch_audit(const DevPtrs &d, void *st) {
 void launch_mb_grid(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
 }
-void launch_mb_update(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL((k_mb_update<KM_ALL>), dim3(d.persist_wgs), dim3(WG), 0, S(st), d);
+void launch_mb_g2p2g(const DevPtrs &d, void *st) {
+  // two resident workgroups per CU (LDS), in the proportion of the single-body grid
+  const int m = d.kmask;
+  const dim3 g(std::max(8, (d.persist_wgs * 2 / 3 + 7) / 8 * 8)), b(WG);
+  const hipStream_t s = S(st);
+  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<true, KM_DP, true>), g, b, 0, s, d);
+  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<true, KM_ELASTIC, true>), g, b, 0, s, d);
+  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<true, KM_ELASTIC | KM_GK, true>), g, b, 0, s, d);
+  else hipLaunchKernelGGL((k_g2p2g<true, KM_ALL, true>), g, b, 0, s, d);
 }
 void launch_mb_cells(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_mb_cells, dim3(std::max(8, std::min(4096, blocks(d.ntiles, WG / TILE3)))), dim3(WG), 0, S(st), d);
