@@ -212,7 +212,8 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
       x.z = run; run += c[4 * q + 2];
       x.w = run; run += c[4 * q + 3];
       ov[q] = x;
-      cv[q] = make_int4(0, 0, 0, 0);
+      // counts are left zero for the next step: only cells that held particles need it
+      if ((c[4 * q] | c[4 * q + 1] | c[4 * q + 2] | c[4 * q + 3]) != 0) cv[q] = make_int4(0, 0, 0, 0);
     }
   }
   // compacted active tile list: heavy tiles (more than one chunk of entries) ascending
