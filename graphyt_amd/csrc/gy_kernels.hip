@@ -1200,15 +1200,26 @@ __global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
     const int down_run = __shfl_down(run, off, 64);
     same[r] = lane + off < 64 && down_run == run;
   }
-  for (int k = 0; k < 3; k++)
+  // the claimed objects (the first two ints of a node's claim) of one z-layer of the
+  // stencil are loaded together, ahead of that layer's atomics: one load latency per layer
+  const int2 *claim2 = reinterpret_cast<const int2 *>(d.claim);
+  for (int k = 0; k < 3; k++) {
+    int2 cls[9];
+#pragma unroll
     for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        cls[3 * j + i] = claim2[2 * (base + i + (long long)I.n[0] * (j + (long long)I.n[1] * k))];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
       for (int i = 0; i < 3; i++) {
         const float A_ = W[1].w[j] * W[2].w[k];
         const float w = W[0].w[i] * A_;
         const float gx = W[0].g[i] * A_, gy = W[0].w[i] * W[1].g[j] * W[2].w[k], gz = W[0].w[i] * W[1].w[j] * W[2].g[k];
         const int g[3] = {b[0] + i, b[1] + j, b[2] + k};
         const long long gi = I.node_base + g[0] + (long long)I.n[0] * (g[1] + (long long)I.n[1] * g[2]);
-        const int4 cl = d.claim[gi];
+        const int2 cl = cls[3 * j + i];
         const bool two = cl.y != -1;
         const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
         const bool use = active && sl >= 0 && (two || rigid_mass);  // uniform over a run
@@ -1239,6 +1250,7 @@ __global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
         unsafeAtomicAdd(&nb[1], v[8]);
         unsafeAtomicAdd(&nb[2], v[9]);
       }
+  }
 }
 
 
