@@ -160,9 +160,25 @@ def bench_ensemble(args) -> int:
 def bench_gk_disk(args) -> int:
     """Brazilian-disk compression with Grady-Kipp damage (1,266,720 particles: disk plus two
     rigid platens, two-body contact step, FP64 eig3 per stressed particle)."""
-    from graphyt_amd import runtime, workloads as W
-    sc = W.gk_disk()
-    log(f"workload gk_disk: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
+    from graphyt_amd import workloads as W
+    return bench_contact(args, W.gk_disk(), "gk_disk: Brazilian disk R 0.05 t 0.02, h 0.001, rigid platens +-0.1 m/s",
+                         "f32 (eig3 and damage f64)")
+
+
+def bench_plane_brazilian(args) -> int:
+    """The plane contact step at about a million particles: the Brazilian test of
+    Example_Scripts/brazillian2D.py (Drucker-Prager rock with Grady-Kipp damage between two
+    rigid platens) at h = 0.125 mm, 1601 x 1601 nodes."""
+    from graphyt_amd import workloads as W
+    return bench_contact(args, W.brazilian_disk_2d(h=0.000125, v=2.0, jitter=0.1),
+                         "plane_brazilian: 2D Brazilian disk R 0.04, h 0.125 mm, DP + Grady-Kipp, rigid platens +-2 m/s",
+                         "f32 (damage f64)")
+
+
+def bench_contact(args, sc, desc, dtype) -> int:
+    """A contact workload: per-kernel HIP-event timing, then the captured hipGraph."""
+    from graphyt_amd import runtime
+    log(f"workload {sc.name}: {sc.n} particles, nodes {sc.dims}, dt {sc.resolved_dt():.4e}")
     ctx = runtime.Context(sc, graph=True)
     ctx.run(args.warmup)
     ctx.synchronize()
@@ -183,10 +199,10 @@ def bench_gk_disk(args) -> int:
     out = {
         "metric": METRIC, "value": round(best, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * sc.n / best, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (eig3 and damage f64)",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": "gk_disk: Brazilian disk R 0.05 t 0.02, h 0.001, rigid platens +-0.1 m/s",
-                   "particles": sc.n, "nodes": list(sc.dims), "dt": sc.resolved_dt(), "parallelism": "single GPU"},
+        "config": {"workload": desc, "particles": sc.n, "nodes": list(sc.dims), "dt": sc.resolved_dt(),
+                   "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in stats.items()},
         "audit_nonfinite": audit["nonfinite"],
@@ -315,7 +331,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
-    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate", "gk_disk", "plane_dam_break"],
+    ap.add_argument("--workload", choices=["dp_column", "ensemble", "surrogate", "gk_disk", "plane_dam_break",
+                                           "plane_brazilian"],
                     default="dp_column",
                     help="dp_column: the 1M-particle column (default); ensemble: 256 batched impacts; "
                          "surrogate: learned vs exact Drucker-Prager return map")
@@ -342,6 +359,8 @@ def main():
         return bench_surrogate(args)
     if args.workload == "gk_disk":
         return bench_gk_disk(args)
+    if args.workload == "plane_brazilian":
+        return bench_plane_brazilian(args)
     if args.workload == "plane_dam_break":
         return bench_plane(args)
     sc = W.dp_column()

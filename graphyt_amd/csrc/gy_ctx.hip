@@ -342,12 +342,11 @@ void enqueue_step(gy_ctx *c) {
   if (c->plane && c->d.multibody) {
     timed(c, K_GRID, [&] { launch_plane_mb_grid(c->d, s); });
     if (th) timed(c, K_THERM_G2P, [&] { launch_therm_g2p(c->d, s); });
-    timed(c, K_G2P2G, [&] { launch_plane_mb_update(c->d, s); });
+    timed(c, K_G2P2G, [&] { launch_plane_mb_g2p2g(c->d, s); });  // with the centre-of-mass P2G
     timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1); });
     timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
     timed(c, K_PLACE, [&] { launch_place(c->d, s); });
     timed(c, K_MB_CLAIM, [&] { launch_mb_cells(c->d, s); launch_plane_mb_claim(c->d, s); });
-    timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_plane_mb_p2g(c->d, s); });
     if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;

@@ -245,7 +245,7 @@ void launch_mb_cells(const DevPtrs &d, void *stream);  // objects per cell (both
 void launch_plane_mb_claim(const DevPtrs &d, void *stream);
 void launch_plane_mb_p2g(const DevPtrs &d, void *stream);
 void launch_plane_mb_grid(const DevPtrs &d, void *stream);
-void launch_plane_mb_update(const DevPtrs &d, void *stream);
+void launch_plane_mb_g2p2g(const DevPtrs &d, void *stream);
 // heat transfer (gy_thermal.hip): node sums of the current state (cleared first), and the
 // particle gradient, rate and temperature update before the fused particle kernel
 void launch_therm_p2g(const DevPtrs &d, void *stream);

@@ -221,8 +221,7 @@ __device__ __noinline__ void plane_p2g_global(float *acc, uint8_t *brick_flag, c
 // `col` of s_rec. Returns the new key (UPDATE) or -1.
 // MB (contact between objects): G2P reads the velocity field of the body slot the
 // particle's object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots'
-// objects; Extrapolation.cpp:288-367), and no record is written (P2G runs after the
-// re-binning).
+// objects; Extrapolation.cpp:288-367). The record is written when s_rec is given.
 template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I, int o0, int o1, double tl, int i,
                                             int slot, const float4 (*s_vel)[2], const MatDev *s_mat, int par,
@@ -405,7 +404,7 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
     if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
   }
 
-  if (MB) return knew;
+  if (s_rec == nullptr) return knew;
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
@@ -414,6 +413,30 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
   w[1] = make_float4(mp * v[1], mp * v[2], nV * s[0], nV * s[1]);
   w[2] = make_float4(nV * s[3], nV * s[4], nV * s[5], 0.0f);
   return knew;
+}
+
+// Contact G2P staging of node n of a plane tile's G2P region: both body slots' velocity
+// and acceleration and the objects that hold them; zero at nodes without mass, which G2P
+// skips (Extrapolation.cpp:290).
+__device__ __forceinline__ void plane_stage_body_vel(const DevPtrs &d, const InstDesc &I, int o0, int o1, int n,
+                                                     float4 (*s_vel)[2], float4 (*s_vel2)[2], int (*s_obj)[PG_N]) {
+  const int g0 = o0 + n % PG_R, g1 = o1 + n / PG_R;
+  float vq[MB_VELB_W];
+  for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
+  int4 cl = make_int4(-1, -1, 0, 0);
+  if (g0 < I.n[0] && g1 < I.n[1]) {
+    const long long ni = I.node_base + g0 + (long long)I.n[0] * g1;
+    if (d.vel[ni * NODE_W + 6] != 0.0f) {
+      for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
+      cl = d.claim[ni];
+    }
+  }
+  s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+  s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
+  s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
+  s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
+  s_obj[0][n] = cl.x;
+  s_obj[1][n] = cl.y;
 }
 
 // Phase B: 9-node scatter of the record in column `col` into the LDS region, or by global
@@ -456,9 +479,16 @@ __device__ __forceinline__ void plane_scatter(const DevPtrs &d, const InstDesc &
 // (P = ceil(n/16) passes), so the entries of one cell, consecutive in the list, are
 // scattered by different passes of phase B (lane = column); then the region is flushed as
 // FP32 partials (core, and the halo shell when reached), node masks and brick flags.
-template <bool UPDATE, int MASK>
+// MB (contact between objects): phase A is the contact G2P on the staged body fields,
+// phase B the centre-of-mass P2G of the new state; the body fields follow the re-binning
+// and the claims (k_plane_mb_p2g).
+template <bool UPDATE, int MASK, bool MB = false>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_plane_g2p2g(DevPtrs d) {
+  static_assert(!MB || UPDATE, "the contact variant moves particles");
+  constexpr int NB2 = MB ? PG_N : 1;
   __shared__ float4 s_vel[PG_N][2];
+  __shared__ float4 s_vel2[NB2][2];
+  __shared__ int s_obj[2][NB2];
   __shared__ __align__(16) double s_acc[7][PACC_N];
   __shared__ PlaneRec s_rec;
   __shared__ int s_off[TILE3 + 1];
@@ -483,7 +513,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     for (int n = tid; n < 7 * PACC_N; n += WG) (&s_acc[0][0])[n] = 0.0;
     if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
     if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
-    if (UPDATE) {
+    if constexpr (MB) {
+      for (int n = tid; n < PG_N; n += WG) plane_stage_body_vel(d, I, o0, o1, n, s_vel, s_vel2, s_obj);
+    } else if (UPDATE) {
       for (int n = tid; n < PG_N; n += WG) {
         const int g0 = o0 + n % PG_R, g1 = o1 + n / PG_R;
         float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
@@ -507,7 +539,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const int j = tid;
       const int col = 16 * (j % P) + j / P;
       int knew = -1;
-      if (i < c1) knew = plane_update<UPDATE, MASK>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col);
+      if (i < c1)
+        knew = plane_update<UPDATE, MASK, MB>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col,
+                                              MB ? s_vel2 : nullptr,
+                                              MB ? reinterpret_cast<const int (*)[PG_N]>(s_obj) : nullptr);
       int hl = 0, hbase = 0;
       if (UPDATE) hbase = hist_issue(d.count, knew, hl);
       __syncthreads();
@@ -702,64 +737,6 @@ __global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
   if (lane == 0) d.brick_flag[gb] = 0;
 }
 
-// G2P + update + models + store + new key on the body-slot velocity fields, persistent over
-// the active tiles (the plane fused kernel without its P2G phase).
-__global__ __launch_bounds__(WG) void k_plane_mb_update(DevPtrs d) {
-  __shared__ float4 s_vel[PG_N][2], s_vel2[PG_N][2];
-  __shared__ int s_obj[2][PG_N];
-  __shared__ MatDev s_mat[MAXMAT];
-  __shared__ int s_off[TILE3 + 1];
-  __shared__ int s_tile;
-  const int tid = threadIdx.x;
-  const int par = d.ctrl->parity;
-  const TileQueue queue(d);
-  for (;;) {
-    if (tid == 0) s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
-    __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(s_tile);
-    if (t < 0) break;
-    const int k0 = t * TILE3;
-    const int ii = d.tile_inst[t];
-    const InstDesc I = d.inst[ii];
-    const int lt = t - I.tile_base;
-    const int o0 = (lt % I.nt[0]) * PT, o1 = (lt / I.nt[0]) * PT;
-    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
-    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
-    for (int n = tid; n < PG_N; n += WG) {
-      const int g0 = o0 + n % PG_R, g1 = o1 + n / PG_R;
-      float vq[MB_VELB_W];
-      for (int f = 0; f < MB_VELB_W; f++) vq[f] = 0.f;
-      int4 cl = make_int4(-1, -1, 0, 0);
-      if (g0 < I.n[0] && g1 < I.n[1]) {
-        const long long ni = I.node_base + g0 + (long long)I.n[0] * g1;
-        if (d.vel[ni * NODE_W + 6] != 0.0f) {  // G2P skips nodes without mass (Extrapolation.cpp:290)
-          for (int f = 0; f < MB_VELB_W; f++) vq[f] = d.velb[ni * MB_VELB_W + f];
-          cl = d.claim[ni];
-        }
-      }
-      s_vel[n][0] = make_float4(vq[0], vq[1], vq[2], vq[3]);
-      s_vel[n][1] = make_float4(vq[4], vq[5], 0.f, 0.f);
-      s_vel2[n][0] = make_float4(vq[6], vq[7], vq[8], vq[9]);
-      s_vel2[n][1] = make_float4(vq[10], vq[11], 0.f, 0.f);
-      s_obj[0][n] = cl.x;
-      s_obj[1][n] = cl.y;
-    }
-    __syncthreads();
-    const double tl = d.t_last[ii];
-    const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
-    for (int c0 = p0; c0 < p1; c0 += WG) {
-      const int i = c0 + tid;
-      int knew = -1;
-      if (i < p1)
-        knew = plane_update<true, KM_ALL, true>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, s_mat, par, nullptr, 0, s_vel2,
-                                                s_obj);
-      const int r = wave_hist_rank(d.count, knew);
-      if (knew >= 0) d.rank[i] = r;
-    }
-    __syncthreads();
-  }
-}
-
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int blocks(long long n, int per) { return (int)((n + per - 1) / per); }
 
@@ -791,8 +768,13 @@ void launch_plane_mb_p2g(const DevPtrs &d, void *st) {
 void launch_plane_mb_grid(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_plane_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
 }
-void launch_plane_mb_update(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_plane_mb_update, dim3(d.persist_wgs), dim3(WG), 0, S(st), d);
+void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
+  const int m = d.kmask;
+  const dim3 g(d.persist_wgs), b(WG);
+  if ((m & ~(KM_DP | KM_GK)) == 0) hipLaunchKernelGGL((k_plane_g2p2g<true, KM_DP | KM_GK, true>), g, b, 0, S(st), d);
+  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0)
+    hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ELASTIC | KM_GK, true>), g, b, 0, S(st), d);
+  else hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ALL, true>), g, b, 0, S(st), d);
 }
 void launch_plane_grid(const DevPtrs &d, void *st) {
   const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
