@@ -615,29 +615,41 @@ __global__ __launch_bounds__(WG) void k_plane_mb_claim(DevPtrs d) {
     if (g0 >= I.n[0] || g1 >= I.n[1]) continue;
     const int Cx = I.n[0] - 1, Cy = I.n[1] - 1;
     const long long ncell = (long long)Cx * Cy;
+    // the 16 cells of the window, all loads issued together (an empty or absent cell
+    // holds no object); then the two passes over them in the window's order
+    int4 co[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int ix = (q & 3) - 2, jy = (q >> 2) - 2;
+      const long long lin = (g0 + ix) + (long long)(g1 + jy) * Cx;
+      int k = -1;
+      if (lin >= 0 && lin < ncell) {
+        const int c0 = (int)(lin % Cx), c1 = (int)(lin / Cx);
+        if (c0 <= I.n[0] - 3 && c1 <= I.n[1] - 3) k = plane_key(I, c0, c1);  // else no particle there
+      }
+      int4 v = make_int4(-1, -1, 0, 0);
+      if (k >= 0) {
+        const int lo = d.offsets[k], hi = d.offsets[k + 1];
+        const int4 c = d.cellobj[k];
+        if (hi != lo) v = c;
+      }
+      co[q] = v;
+    }
     int b1 = -1, b2 = -1, r1 = 0, r2 = 0;
-    for (int pass = 0; pass < 2; pass++)
-      for (int jy = -2; jy < 2; jy++)
-        for (int ix = -2; ix < 2; ix++) {
-          const long long lin = (g0 + ix) + (long long)(g1 + jy) * Cx;
-          if (lin < 0 || lin >= ncell) continue;
-          const int c0 = (int)(lin % Cx), c1 = (int)(lin / Cx);
-          if (c0 > I.n[0] - 3 || c1 > I.n[1] - 3) continue;  // no particle there
-          const int k = plane_key(I, c0, c1);
-          if (d.offsets[k + 1] == d.offsets[k]) continue;
-          const int4 co = d.cellobj[k];
-          if (pass == 0) {
-            const int ob[2] = {co.x, co.y};
-            for (int q = 0; q < 2; q++) {
-              if (ob[q] < 0) continue;
-              if (b1 < 0) b1 = ob[q];
-              else if (b2 < 0 && ob[q] != b1) b2 = ob[q];
-            }
-          } else {
-            r1 |= (co.x == b1 && co.z) || (co.y == b1 && co.w);
-            r2 |= b2 >= 0 && ((co.x == b2 && co.z) || (co.y == b2 && co.w));
-          }
-        }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int ob[2] = {co[q].x, co[q].y};
+      for (int u = 0; u < 2; u++) {
+        if (ob[u] < 0) continue;
+        if (b1 < 0) b1 = ob[u];
+        else if (b2 < 0 && ob[u] != b1) b2 = ob[u];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      r1 |= (co[q].x == b1 && co[q].z) || (co[q].y == b1 && co[q].w);
+      r2 |= b2 >= 0 && ((co[q].x == b2 && co[q].z) || (co[q].y == b2 && co[q].w));
+    }
     d.claim[I.node_base + g0 + (long long)I.n[0] * g1] = make_int4(b1, b2, r1, r2);
   }
 }
