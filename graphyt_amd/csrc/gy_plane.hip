@@ -660,47 +660,83 @@ __global__ __launch_bounds__(WG) void k_plane_mb_claim(DevPtrs d) {
 // (Extrapolation.cpp:210-256). FP32 global atomics.
 __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
-  if (s >= d.np) return;
+  const int lane = threadIdx.x & 63;
+  const bool live = s < d.np;
+  const int sc = live ? s : d.np - 1;  // dead lanes read a valid slot and contribute nothing
   const uint32_t *A = (d.ctrl->parity ? d.pbuf[1] : d.pbuf[0]);
   const long long cap = d.cap;
-  const InstDesc &I = d.inst[inst_of_slot(d, s)];
-  const uint32_t tag = A[F_TAG * cap + s];
+  const InstDesc &I = d.inst[inst_of_slot(d, sc)];
+  const uint32_t tag = A[F_TAG * cap + sc];
   const MatDev &M = d.mats[I.mat_base + tag_mat(tag)];
   const int obj = tag_obj(tag);
   bool bad = false;
-  const int q0 = (int)A[F_QX * cap + s], q1 = (int)A[F_QY * cap + s];
+  const int q0 = (int)A[F_QX * cap + sc], q1 = (int)A[F_QY * cap + sc];
   const int b0 = stencil_base(q0, I.lpq, I.fbits, I.n[0] - 3, bad);
   const int b1 = stencil_base(q1, I.lpq, I.fbits, I.n[1] - 3, bad);
   const Axis W0 = gimp3((float)(q0 - (b0 << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
   const Axis W1 = gimp3((float)(q1 - (b1 << I.fbits)) * I.inv_2F, I.lp, I.inv_lp, I.inv_h);
-  const float mass = f32(A[F_MASS * cap + s]), rho = f32(A[F_RHO * cap + s]);
+  const float mass = f32(A[F_MASS * cap + sc]), rho = f32(A[F_RHO * cap + sc]);
   const float mp = M.rigid ? 1.0f : mass;
   const float nV = -(mp / rho), Vn = mass / rho;
   float mv[3], vs[6];
-  for (int a = 0; a < 3; a++) mv[a] = mp * f32(A[(F_VX + a) * cap + s]);
-  for (int a = 0; a < 6; a++) vs[a] = nV * f32(A[(F_S0 + a) * cap + s]);
+  for (int a = 0; a < 3; a++) mv[a] = mp * f32(A[(F_VX + a) * cap + sc]);
+  for (int a = 0; a < 6; a++) vs[a] = nV * f32(A[(F_S0 + a) * cap + sc]);
+  // Adjacent lanes with the same stencil base node, object and rigid flag share all 9
+  // target nodes and body slots: their terms are summed over the run (segmented shuffle
+  // reduction) and the run's first lane issues the atomics. Runs are numbered by the heads
+  // at or below a lane; every shuffle runs with the whole wave active.
+  const int base = (int)(I.node_base + b0 + (long long)I.n[0] * b1);
+  const int seg_base = live ? base : -1 - lane;
+  const int seg_obj = obj * 2 + (M.rigid ? 1 : 0);
+  const int up_base = __shfl_up(seg_base, 1, 64), up_obj = __shfl_up(seg_obj, 1, 64);
+  const bool head = lane == 0 || up_base != seg_base || up_obj != seg_obj;
+  const unsigned long long heads = __ballot(head);
+  const int run = __popcll(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull)));
+  bool same[6];
+#pragma unroll
+  for (int r = 0, off = 1; r < 6; r++, off <<= 1) {
+    const int down_run = __shfl_down(run, off, 64);
+    same[r] = lane + off < 64 && down_run == run;
+  }
+  // the claimed objects of the 9 stencil nodes, loaded together
+  const int2 *claim2 = reinterpret_cast<const int2 *>(d.claim);
+  int2 cls[9];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) cls[3 * j + i] = claim2[2 * (base + i + (long long)I.n[0] * j)];
+#pragma unroll
   for (int j = 0; j < 3; j++)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const float w = W0.w[i] * W1.w[j], gx = W0.g[i] * W1.w[j], gy = W0.w[i] * W1.g[j];
-      if (w == 0.0f && gx == 0.0f && gy == 0.0f) continue;
-      const long long gi = I.node_base + (b0 + i) + (long long)I.n[0] * (b1 + j);
-      const int4 cl = d.claim[gi];
+      const int2 cl = cls[3 * j + i];
       const bool two = cl.y != -1;
-      if (!two && !M.rigid) continue;
       const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
-      if (sl < 0) continue;
+      const bool use = live && sl >= 0 && (two || M.rigid);  // uniform over a run
+      if (__ballot(use) == 0ull) continue;  // wave-uniform: nothing to add at this node
+      float v[10] = {w * mp, w * mv[0], w * mv[1], w * mv[2], vs[0] * gx + vs[3] * gy, vs[3] * gx + vs[1] * gy,
+                     vs[5] * gx + vs[4] * gy, gx * Vn, gy * Vn, w * mp};
+#pragma unroll
+      for (int f = 0; f < 10; f++) v[f] = use ? v[f] : 0.0f;
+#pragma unroll
+      for (int r = 0, off = 1; r < 6; r++, off <<= 1)
+#pragma unroll
+        for (int f = 0; f < 10; f++) {
+          const float o = __shfl_down(v[f], off, 64);
+          v[f] += same[r] ? o : 0.0f;
+        }
+      if (!(use && head)) continue;
       int c0, c1;
       const long long ci = plane_canon(I, b0 + i, b1 + j, c0, c1);
       float *ab = d.acc2 + ci * MB_ACC2_W + sl * 8;
-      if (M.rigid) unsafeAtomicAdd(&ab[7], w * mp);
+      if (M.rigid) unsafeAtomicAdd(&ab[7], v[9]);
       if (!two) continue;
-      const float t[7] = {w * mp, w * mv[0], w * mv[1], w * mv[2], vs[0] * gx + vs[3] * gy, vs[3] * gx + vs[1] * gy,
-                          vs[5] * gx + vs[4] * gy};
-      for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], t[f]);
+      for (int f = 0; f < 7; f++) unsafeAtomicAdd(&ab[f], v[f]);
+      const long long gi = I.node_base + (b0 + i) + (long long)I.n[0] * (b1 + j);
       float *nb = d.nrm + gi * 6 + sl * 3;
-      unsafeAtomicAdd(&nb[0], gx * Vn);
-      unsafeAtomicAdd(&nb[1], gy * Vn);
+      unsafeAtomicAdd(&nb[0], v[7]);
+      unsafeAtomicAdd(&nb[1], v[8]);
     }
 }
 
