@@ -394,6 +394,7 @@ void enqueue_init(gy_ctx *c) {
       timed(c, K_MB_CLAIM, [&] { launch_mb_cells(c->d, s); launch_plane_mb_claim(c->d, s); });
       timed(c, K_P2G, [&] { launch_plane_p2g_only(c->d, s); });
       timed(c, K_MB_P2G, [&] { launch_plane_mb_p2g(c->d, s); });
+      launch_brick_list(c->d, s);
       if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
       return;
     }
@@ -410,6 +411,7 @@ void enqueue_init(gy_ctx *c) {
     timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
     timed(c, K_P2G, [&] { launch_p2g_only(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
+    launch_brick_list(c->d, s);
     if (c->d.thermal) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
     return;
   }

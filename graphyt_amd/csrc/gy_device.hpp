@@ -196,6 +196,24 @@ __device__ __forceinline__ void contact_normal(const float rel[3], const float n
   for (int a = 0; a < 3; a++) cn[a] = n[a] + m * on[a];
 }
 
+// The direct-scatter centre-of-mass sums of canonical node ci (added to a7) and its body
+// sums (into ab), read and cleared for the next step with 16-byte accesses.
+__device__ __forceinline__ void take_node_sums(const DevPtrs &d, long long ci, float a7[7], float ab[MB_ACC2_W]) {
+  float4 *ap = reinterpret_cast<float4 *>(d.acc + ci * NODE_W);
+  const float4 x0 = ap[0], x1 = ap[1];
+  a7[0] += x0.x; a7[1] += x0.y; a7[2] += x0.z; a7[3] += x0.w;
+  a7[4] += x1.x; a7[5] += x1.y; a7[6] += x1.z;
+  ap[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+  ap[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 *bp = reinterpret_cast<float4 *>(d.acc2 + ci * MB_ACC2_W);
+#pragma unroll
+  for (int q = 0; q < MB_ACC2_W / 4; q++) {
+    const float4 b = bp[q];
+    ab[4 * q] = b.x; ab[4 * q + 1] = b.y; ab[4 * q + 2] = b.z; ab[4 * q + 3] = b.w;
+    bp[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 // Contact solve of one actual node (Grid.cpp:9-245) and its BCs on the centre of mass and
 // both bodies (Grid.cpp:967-982, 1012-1031): m, a7 the centre-of-mass sums of its periodic
 // class (m, p, f_int), ab the body sums [m, p, f_int, rigid mass] x 2, vc and ac the

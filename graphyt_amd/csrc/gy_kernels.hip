@@ -1259,17 +1259,11 @@ __global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
 // claims and normals (Grid.cpp:9-245), BCs on the centre of mass and both bodies
 // (Grid.cpp:967-1031). VEL gets the centre-of-mass velocity p/m, its acceleration and m;
 // VELB the body velocities and accelerations G2P reads.
-__global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gb = blockIdx.x * (WG / 64) + wave;
-  if (gb >= d.nbricks) return;
-  if (!d.brick_flag[gb]) return;
+__device__ __forceinline__ void mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
   const int c0 = bx * BRICK + (lane & 3), c1 = by * BRICK + ((lane >> 2) & 3), c2 = bz * BRICK + (lane >> 4);
-  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
   if (c0 < I.cper[0] && c1 < I.cper[1] && c2 < I.cper[2]) {
     const long long ci = I.node_base + c0 + (long long)I.n[0] * (c1 + (long long)I.n[1] * c2);
     // centre-of-mass sums: tile partials of the periodic class and the direct scatter
@@ -1284,8 +1278,7 @@ __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
         }
       }
     }
-    for (int f = 0; f < 7; f++) { a7[f] += d.acc[ci * NODE_W + f]; d.acc[ci * NODE_W + f] = 0.f; }
-    for (int f = 0; f < MB_ACC2_W; f++) { ab[f] = d.acc2[ci * MB_ACC2_W + f]; d.acc2[ci * MB_ACC2_W + f] = 0.f; }
+    take_node_sums(d, ci, a7, ab);
     const float m = a7[0];
     const float dt = I.dt, kick = (step == 1) ? 0.5f * dt : dt;
     float vc[3] = {0.f, 0.f, 0.f}, ac[3] = {0.f, 0.f, 0.f};
@@ -1310,6 +1303,16 @@ __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
     }
   }
   if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// Contact grid update, persistent over the list of flagged bricks (one wave per brick).
+__global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
+  // the particle kernel that follows walks the active tile list from the start
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
+    mb_grid_brick(d, d.brick_list[k], lane, step, epoch);
 }
 
 // FP64 audit: per block sums of mass, momentum, kinetic energy, a stress-energy proxy and
@@ -1422,7 +1425,8 @@ void launch_audit(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_audit_final, dim3(1), dim3(WG), 0, S(st), d, nb);
 }
 void launch_mb_grid(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  hipLaunchKernelGGL(k_mb_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 void launch_mb_g2p2g(const DevPtrs &d, void *st) {
   // two resident workgroups per CU (LDS), in the proportion of the single-body grid

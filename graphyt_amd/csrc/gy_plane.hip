@@ -743,16 +743,10 @@ __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
 // Contact nodal solve on the flagged plane bricks (one wave per 8 x 8 brick): centre-of-mass
 // sums of the periodic class (tile partials and direct scatter), body sums, then per actual
 // node its own claims, normals, solve and BCs (mb_solve_node).
-__global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gb = blockIdx.x * (WG / 64) + wave;
-  if (gb >= d.nbricks) return;
-  if (!d.brick_flag[gb]) return;
+__device__ __forceinline__ void plane_mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int c0 = (lb % I.nb[0]) * PB + (lane & 7), c1 = (lb / I.nb[0]) * PB + (lane >> 3);
-  const int step = d.ctrl->step, epoch = d.ctrl->epoch;
   if (c0 < I.cper[0] && c1 < I.cper[1]) {
     const long long ci = I.node_base + c0 + (long long)I.n[0] * c1;
     float a7[7] = {0, 0, 0, 0, 0, 0, 0}, ab[MB_ACC2_W];
@@ -763,8 +757,7 @@ __global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
         plane_gather(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], a7);
       }
     }
-    for (int f = 0; f < 7; f++) { a7[f] += d.acc[ci * NODE_W + f]; d.acc[ci * NODE_W + f] = 0.f; }
-    for (int f = 0; f < MB_ACC2_W; f++) { ab[f] = d.acc2[ci * MB_ACC2_W + f]; d.acc2[ci * MB_ACC2_W + f] = 0.f; }
+    take_node_sums(d, ci, a7, ab);
     const float m = a7[0];
     const float dt = I.dt, kick = (step == 1) ? 0.5f * dt : dt;
     float vc[3] = {0.f, 0.f, 0.f}, ac[3] = {0.f, 0.f, 0.f};
@@ -783,6 +776,16 @@ __global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
     }
   }
   if (lane == 0) d.brick_flag[gb] = 0;
+}
+
+// Plane contact grid update, persistent over the list of flagged bricks (one wave per brick).
+__global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
+  // the particle kernel that follows walks the active tile list from the start
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
+    plane_mb_grid_brick(d, d.brick_list[k], lane, step, epoch);
 }
 
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
@@ -814,7 +817,8 @@ void launch_plane_mb_p2g(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_plane_mb_p2g, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
 void launch_plane_mb_grid(const DevPtrs &d, void *st) {
-  hipLaunchKernelGGL(k_plane_mb_grid, dim3(blocks(d.nbricks, WG / 64)), dim3(WG), 0, S(st), d);
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  hipLaunchKernelGGL(k_plane_mb_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
   const int m = d.kmask;
