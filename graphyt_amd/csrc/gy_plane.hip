@@ -823,7 +823,9 @@ void launch_plane_mb_grid(const DevPtrs &d, void *st) {
 void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
   const int m = d.kmask;
   const dim3 g(d.persist_wgs), b(WG);
-  if ((m & ~(KM_DP | KM_GK)) == 0) hipLaunchKernelGGL((k_plane_g2p2g<true, KM_DP | KM_GK, true>), g, b, 0, S(st), d);
+  // rigid platens count as elastic: a damaging Drucker-Prager body between them is 1 + 8 + 16
+  if ((m & ~(KM_ELASTIC | KM_DP | KM_GK)) == 0)
+    hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ELASTIC | KM_DP | KM_GK, true>), g, b, 0, S(st), d);
   else if ((m & ~(KM_ELASTIC | KM_GK)) == 0)
     hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ELASTIC | KM_GK, true>), g, b, 0, S(st), d);
   else hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ALL, true>), g, b, 0, S(st), d);
