@@ -65,34 +65,39 @@ __global__ __launch_bounds__(WG) void k_keys(DevPtrs d) {
   if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
 }
 
-// Block-wide exclusive scan of one int per thread (wave64 shuffles + LDS).
-__device__ __forceinline__ int block_exclusive_scan(int v, int &total) {
-  __shared__ int wsum[WG / 64];
+// N independent block-wide exclusive scans in one pass (one shuffle chain per value, one
+// pair of barriers for all): v[k] becomes its exclusive prefix, total[k] the block total.
+template <int N>
+__device__ __forceinline__ void block_scan_n(int (&v)[N], int (&total)[N]) {
+  __shared__ int wsum[N][WG / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int x = v;
+  int x[N];
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) wsum[w] = x;
-  __syncthreads();
-  int before = 0, all = 0;
+  for (int k = 0; k < N; k++) x[k] = v[k];
 #pragma unroll
-  for (int k = 0; k < WG / 64; k++) {
-    const int t = wsum[k];
-    if (k < w) before += t;
-    all += t;
+  for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const int y = __shfl_up(x[k], o, 64);
+      if (lane >= o) x[k] += y;
+    }
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < N; k++) wsum[k][w] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    int before = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < WG / 64; q++) {
+      const int t = wsum[k][q];
+      if (q < w) before += t;
+      all += t;
+    }
+    total[k] = all;
+    v[k] = before + x[k] - v[k];
   }
   __syncthreads();
-  total = all;
-  return before + x - v;
-}
-
-__device__ __forceinline__ int block_sum(int v) {
-  int tot;
-  block_exclusive_scan(v, tot);
-  return tot;
 }
 
 __global__ __launch_bounds__(WG) void k_hist(DevPtrs d) {
@@ -162,9 +167,9 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
   int ts = s + __shfl_xor(s, 1, 64);
   ts += __shfl_xor(ts, 2, 64);
   const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
-  const int tot = block_sum(s);
-  const int tact = block_sum(act);
-  const int theavy = block_sum((act && ts > CHUNK_ENTRIES) ? 1 : 0);
+  int v3[3] = {s, act, (act && ts > CHUNK_ENTRIES) ? 1 : 0}, t3[3];
+  block_scan_n(v3, t3);
+  const int tot = t3[0], tact = t3[1], theavy = t3[2];
   if (threadIdx.x == 0) {
     d.blocksum[blockIdx.x] = tot;
     d.blocktiles[blockIdx.x] = tact | (theavy << 16);  // at most SCAN_BLOCK / TILE3 each
@@ -184,11 +189,11 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     allt += bt;
     allh += bh;
   }
-  pre = block_sum(pre);
-  pret = block_sum(pret);
-  preh = block_sum(preh);
-  allt = block_sum(allt);
-  allh = block_sum(allh);
+  {
+    int v5[5] = {pre, pret, preh, allt, allh}, t5[5];
+    block_scan_n(v5, t5);
+    pre = t5[0]; pret = t5[1]; preh = t5[2]; allt = t5[3]; allh = t5[4];
+  }
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   const bool have = base < d.nkeys;
   int c[SCAN_ITEMS];
@@ -199,8 +204,15 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     if (!have) c[j] = 0;
     s += c[j];
   }
-  int tot;
-  int run = pre + block_exclusive_scan(s, tot);
+  // exclusive in-block prefixes of the particle count and of the active and heavy tiles
+  int ts = s + __shfl_xor(s, 1, 64);
+  ts += __shfl_xor(ts, 2, 64);
+  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
+  const int heavy = (act && ts > CHUNK_ENTRIES) ? 1 : 0;
+  int v3[3] = {s, act, heavy}, t3[3];
+  block_scan_n(v3, t3);
+  const int tot = t3[0], ttot = t3[1];
+  int run = pre + v3[0];
   if (have) {
     int4 *ov = reinterpret_cast<int4 *>(d.offsets + base);
     int4 *cv = reinterpret_cast<int4 *>(d.count + base);
@@ -218,13 +230,8 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   }
   // compacted active tile list: heavy tiles (more than one chunk of entries) ascending
   // from the front, light ones ascending from the back
-  int ts = s + __shfl_xor(s, 1, 64);
-  ts += __shfl_xor(ts, 2, 64);
-  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
-  const int heavy = (act && ts > CHUNK_ENTRIES) ? 1 : 0;
-  int ttot, htot;
-  const int pos = pret + block_exclusive_scan(act, ttot);  // index in tile order
-  const int hpos = preh + block_exclusive_scan(heavy, htot);  // heavy tiles before it
+  const int pos = pret + v3[1];   // index in tile order
+  const int hpos = preh + v3[2];  // heavy tiles before it
   if (act) d.active[heavy ? hpos : allt - 1 - (pos - hpos)] = (int)(base / TILE3);
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allh;
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
