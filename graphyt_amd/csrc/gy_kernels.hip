@@ -1003,6 +1003,7 @@ __global__ __launch_bounds__(WG) void k_mb_cells(DevPtrs d) {
     const int t = d.active[ta];
     const InstDesc &I = d.inst[d.tile_inst[t]];
     const int k = t * TILE3 + (threadIdx.x % TILE3);
+    if (threadIdx.x % TILE3 == 0) d.twotile[t] = 0;  // set again by the claims that follow
     int a = 0x7fffffff, b = 0x7fffffff, ra = 0, rb = 0;
     for (int j = d.offsets[k]; j < d.offsets[k + 1]; j++) {
       const uint32_t tag = A[F_TAG * d.cap + d.tmp[j]];

@@ -651,6 +651,7 @@ __global__ __launch_bounds__(WG) void k_plane_mb_claim(DevPtrs d) {
       r2 |= b2 >= 0 && ((co[q].x == b2 && co[q].z) || (co[q].y == b2 && co[q].w));
     }
     d.claim[I.node_base + g0 + (long long)I.n[0] * g1] = make_int4(b1, b2, r1, r2);
+    if (b2 >= 0) d.twotile[t] = 1;  // the tile's particles have a two-body node in reach
   }
 }
 
@@ -668,6 +669,11 @@ __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
   const InstDesc &I = d.inst[inst_of_slot(d, sc)];
   const uint32_t tag = A[F_TAG * cap + sc];
   const MatDev &M = d.mats[I.mat_base + tag_mat(tag)];
+  // a particle's 9 nodes lie in its tile's claim region: without a two-body node there
+  // only rigid particles add (their mass to their slot)
+  const int key = (d.ctrl->parity ? d.key[1] : d.key[0])[sc];
+  const bool active = live && (M.rigid || d.twotile[key / TILE3]);
+  if (__ballot(active) == 0ull) return;  // wave-uniform
   const int obj = tag_obj(tag);
   bool bad = false;
   const int q0 = (int)A[F_QX * cap + sc], q1 = (int)A[F_QY * cap + sc];
@@ -686,7 +692,7 @@ __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
   // reduction) and the run's first lane issues the atomics. Runs are numbered by the heads
   // at or below a lane; every shuffle runs with the whole wave active.
   const int base = (int)(I.node_base + b0 + (long long)I.n[0] * b1);
-  const int seg_base = live ? base : -1 - lane;
+  const int seg_base = active ? base : -1 - lane;
   const int seg_obj = obj * 2 + (M.rigid ? 1 : 0);
   const int up_base = __shfl_up(seg_base, 1, 64), up_obj = __shfl_up(seg_obj, 1, 64);
   const bool head = lane == 0 || up_base != seg_base || up_obj != seg_obj;
@@ -713,7 +719,7 @@ __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
       const int2 cl = cls[3 * j + i];
       const bool two = cl.y != -1;
       const int sl = (cl.x == obj) ? 0 : (cl.y == obj) ? 1 : -1;
-      const bool use = live && sl >= 0 && (two || M.rigid);  // uniform over a run
+      const bool use = active && sl >= 0 && (two || M.rigid);  // uniform over a run
       if (__ballot(use) == 0ull) continue;  // wave-uniform: nothing to add at this node
       float v[10] = {w * mp, w * mv[0], w * mv[1], w * mv[2], vs[0] * gx + vs[3] * gy, vs[3] * gx + vs[1] * gy,
                      vs[5] * gx + vs[4] * gy, gx * Vn, gy * Vn, w * mp};
