@@ -18,6 +18,15 @@ using namespace gy;
 
 namespace {
 
+// bfloat16 of x, round to nearest even, in the high half of a word (the velocity residual
+// of the F_XV words)
+uint32_t bf16_bits_rn(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u & 0xffff0000u;
+}
+
 const char *kKernelNames[K_COUNT_] = {"keys",  "scan_partial", "scan_final", "place",   "p2g",
                                       "grid",  "g2p2g",        "advance",    "audit",   "order",
                                       "mb_claim", "mb_p2g", "therm_p2g", "therm_g2p"};
@@ -587,11 +596,20 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
       const long long s_ = pstart + i;
       for (int a = 0; a < 3; a++) {
         const double x = in->pos[3 * i + a];
-        const double q = std::nearbyint(x * scale);
-        if (!(std::fabs(q) < 2.0e9)) return fail(c, GY_ERR_RANGE, "instance %d particle %d: position out of the fixed-point range", k, i);
-        F(F_QX + a, s_) = (uint32_t)(int32_t)q;
+        // 48-bit fixed point: q = the quantum count, lo = the fraction below it in 2^-16
+        const double Q = std::nearbyint(x * scale * 65536.0);
+        if (!(std::fabs(Q) < 2.0e9 * 65536.0)) return fail(c, GY_ERR_RANGE, "instance %d particle %d: position out of the fixed-point range", k, i);
+        const long long Qi = (long long)Q;
+        F(F_QX + a, s_) = (uint32_t)(int32_t)(Qi >> 16);
+        uint32_t xv = (uint32_t)(Qi & 0xffff);
         H.pos0[3 * i + a] = in->pos0 ? in->pos0[3 * i + a] : x;
-        if (in->vel) put(F_VX + a, s_, in->vel[3 * i + a]);
+        if (in->vel) {  // v = v_hi + residual, the residual as a bfloat16
+          const double v = in->vel[3 * i + a];
+          const float vh = (float)v;
+          put(F_VX + a, s_, v);
+          xv |= bf16_bits_rn((float)(v - (double)vh));
+        }
+        F(F_XV0 + a, s_) = xv;
       }
       const int m = in->material ? in->material[i] : 0;
       for (int kk = 0; kk < 6; kk++) {
@@ -767,12 +785,18 @@ static int download_impl(gy_ctx *c, int inst, gy_particles *o) {
     const uint32_t tag = U(F_TAG, s);
     const int m = tag_mat(tag);
     double x[3];
-    for (int a = 0; a < 3; a++) x[a] = (double)(int32_t)U(F_QX + a, s) * inv;
+    for (int a = 0; a < 3; a++)
+      x[a] = ((double)(int32_t)U(F_QX + a, s) + (double)(U(F_XV0 + a, s) & 0xffffu) * (1.0 / 65536.0)) * inv;
     for (int a = 0; a < 3; a++) {
       if (o->pos) o->pos[3 * i + a] = x[a];
       if (o->pos0) o->pos0[3 * i + a] = H.pos0[3 * i + a];
       if (o->displacement) o->displacement[3 * i + a] = x[a] - H.pos0[3 * i + a];
-      if (o->vel) o->vel[3 * i + a] = Fv(F_VX + a, s);
+      if (o->vel) {
+        const uint32_t r = U(F_XV0 + a, s) & 0xffff0000u;
+        float vr;
+        std::memcpy(&vr, &r, 4);
+        o->vel[3 * i + a] = Fv(F_VX + a, s) + (double)vr;
+      }
     }
     const bool cp = H.carry_plastic[m], cd = H.carry_damage[m];
     for (int k = 0; k < 6; k++) {

@@ -24,6 +24,35 @@ __device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
 // -12 %, step -3 %); the plane kernel reads plain (there streaming measured 6 % slower).
 __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p) { return __builtin_nontemporal_load(p); }
 
+// Extended position and velocity (the F_XV words). A position is the 48-bit fixed-point
+// number (q << 16) + lo in units of 2^-16 quantum; a step advances it by d quanta with one
+// rounding at 2^-16 quantum. A velocity is v_hi + r with r held as a bfloat16 (8 bits below
+// v_hi's last bit); vel_add adds an increment as an error-free sum of v_hi and the
+// increment plus the carried residual (Knuth's two-sum, then a renormalisation), so the
+// rounding of each step's update stays at ~2^-32 of v instead of 2^-24.
+__device__ __forceinline__ uint32_t xv_lo(uint32_t w) { return w & 0xffffu; }
+__device__ __forceinline__ float xv_res(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ uint32_t xv_pack(uint32_t lo, float res) {
+  uint32_t u = __float_as_uint(res);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even at bfloat16
+  return (u & 0xffff0000u) | (lo & 0xffffu);
+}
+__device__ __forceinline__ void advance_q(int &q, uint32_t &lo, float dquanta) {
+  const long long Q = ((long long)q << 16) + (long long)lo + __float2ll_rn(dquanta * 65536.0f);
+  q = (int)(Q >> 16);
+  lo = (uint32_t)(Q & 0xffff);
+}
+__device__ __forceinline__ void vel_add(float &vh, float &vr, float inc) {
+#pragma clang fp contract(off)
+  const float s = vh + inc;
+  const float bb = s - vh;
+  const float err = (vh - (s - bb)) + (inc - bb);
+  const float lo = vr + err;
+  const float nh = s + lo;
+  vr = lo - (nh - s);
+  vh = nh;
+}
+
 // Stencil base cell of a fixed-point coordinate, clamped to the grid (getGIMPNodes,
 // Extrapolation.cpp:658-660: int((x - psep/2 - x0)/h)); `bad` is set when clamped.
 __device__ __forceinline__ int stencil_base(int q, int lpq, int F, int maxb, bool &bad) {

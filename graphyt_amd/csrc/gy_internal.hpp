@@ -4,7 +4,10 @@
 // base[f * cap + s] (fields listed in PField). Positions are per-axis fixed point:
 // q = x / h * 2^F (int32), so a particle's cell and its offset inside the cell are exact
 // integers/floats and single-precision round-off never depends on the distance from the
-// origin. Two state buffers (A, B) alternate every step: the fused particle kernel reads
+// origin. Position and velocity are the two fields that accumulate a small increment into
+// a large value every step, so each carries 16 more bits per axis (F_XV0..2): the position
+// fraction below the quantum h/2^F (units of 2^-16 quantum, low half) and the bfloat16 of
+// the velocity residual v - v_hi (high half), v = v_hi + residual to about 32 bits. Two state buffers (A, B) alternate every step: the fused particle kernel reads
 // slot order of one binning and writes the other in cell-sorted order.
 //
 // Grid: per node 8 floats (AoS, 32 B). ACC holds P2G sums (m, px, py, pz, fx, fy, fz, -)
@@ -64,6 +67,7 @@ enum PField : int {
   F_PID,                                    // caller's particle index
   F_T,                                      // temperature
   F_TG0, F_TG1, F_TG2,                      // temperature gradient of the last G2P
+  F_XV0, F_XV1, F_XV2,                      // per axis: position fraction (low 16) | velocity residual bf16 (high 16)
   NF
 };
 

@@ -487,13 +487,15 @@ struct RecLds {
 // constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
 // state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
 // Node velocity and acceleration of the staged G2P region: s_vel[n] = {(vx, vy, vz, ax),
-// (ay, az, -, -)}, two 16-byte LDS reads per node.
+// (ay, az, -, -)}, two 16-byte LDS reads per node; the velocity relative to the tile's
+// reference velocity vref (zero in the contact variant).
 // MB (two-body contact): G2P reads the velocity field of the body slot the particle's
 // object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots' objects;
 // Extrapolation.cpp:288-367). The record is written when s_rec is given.
 template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
-                                               int i, int slot, const float4 (*s_vel)[2], const MatDev *s_mat,
+                                               int i, int slot, const float4 (*s_vel)[2], float3 vref,
+                                               const MatDev *s_mat,
                                                int par, RecLds *s_rec, int j,
                                                const float4 (*s_vel2)[2] = nullptr,
                                                int (*s_obj)[NODES_G2P] = nullptr) {
@@ -516,6 +518,9 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
 
   if (UPDATE) {
     const int out = i;
+    uint32_t xv[3];  // position fraction | velocity residual (gy_internal.hpp)
+#pragma unroll
+    for (int a = 0; a < 3; a++) xv[a] = ld_stream(&A[(F_XV0 + a) * cap + slot]);
     bool bad = false;
     int b[3];
     Axis W[3];
@@ -577,6 +582,9 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         }
       }
     }
+    vg[0] += vref.x;
+    vg[1] += vref.y;
+    vg[2] += vref.z;
     float er[6];
     er[0] = Lx[0];
     er[1] = Ly[1];
@@ -593,7 +601,11 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       if (bct) {
 #pragma unroll
         for (int k = 0; k < 4; k++) bv[k] = d.pbc_val[4 * (long long)pid + k];
+        const float v0[3] = {v[0], v[1], v[2]};
         particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 4 (Solver.cpp:219-223)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (v[a] != v0[a]) xv[a] &= 0xffffu;  // a prescribed velocity has no residual
       }
     }
     // ---- particle update (TimeIntegration.cpp:11-117) ----
@@ -602,8 +614,11 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     if (!tag_hold(tag)) {
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        v[a] = v[a] + (ap[a] - I.damping * v[a]) * dt;
-        q[a] += __float2int_rn(vg[a] * I.qscale);
+        float vr = xv_res(xv[a]);
+        uint32_t lo = xv_lo(xv[a]);
+        vel_add(v[a], vr, (ap[a] - I.damping * v[a]) * dt);
+        advance_q(q[a], lo, vg[a] * I.qscale);
+        xv[a] = xv_pack(lo, vr);
       }
     }
     float e[6];
@@ -663,7 +678,13 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
       if (q[a] < I.oneq) q[a] += I.wrapq[a] - I.oneq;
     }
     if (MASK & KM_PBC) {
-      if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+      if (bct) {
+        const float v0[3] = {v[0], v[1], v[2]};
+        particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (v[a] != v0[a]) xv[a] &= 0xffffu;
+      }
     }
 
     // ---- store the new state at its sorted position ----
@@ -671,6 +692,8 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     for (int a = 0; a < 3; a++) B[(F_QX + a) * cap + out] = (uint32_t)q[a];
 #pragma unroll
     for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) B[(F_XV0 + a) * cap + out] = xv[a];
 #pragma unroll
     for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
     B[F_RHO * cap + out] = u32(rho);
@@ -831,6 +854,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))
   __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
   __shared__ int s_tile;
+  __shared__ int s_refkey[WG / 64];
+  __shared__ float4 s_refv[WG / 64];
   const int tid = threadIdx.x;
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
@@ -850,6 +875,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))
     const InstDesc I = d.inst[ii];
     const int lt = t - I.tile_base;
     const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
+    float3 vref = make_float3(0.f, 0.f, 0.f);  // staged node velocities are relative to vref
 
     {
       double2 *z = reinterpret_cast<double2 *>(&s_acc[0][0]);
@@ -860,17 +886,47 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))
     if constexpr (MB) {
       for (int n = tid; n < NODES_G2P; n += WG) stage_body_vel(d, I, o, n, s_vel, s_vel2, s_obj);
     } else if (UPDATE) {
-      for (int n = tid; n < NODES_G2P; n += WG) {
+      // Stage the region's node velocity and acceleration; nodes without mass stage zero
+      // (G2P skips them, Extrapolation.cpp:290). The velocities are then staged relative to
+      // a reference velocity of the tile, the massive node nearest the region centre: the
+      // 27 GIMP weights of an interior stencil sum to 1 and their gradients to 0, so
+      // sum(N v) = v_ref + sum(N (v - v_ref)) and sum(gradN v) = sum(gradN (v - v_ref)),
+      // and the FP32 sums then round the small differences instead of the full velocities
+      // (the velocity gradient of a slowly deforming, fast moving body is otherwise lost
+      // to cancellation).
+      static_assert(NODES_G2P <= WG, "one staged node per thread");
+      const int n = tid;
+      float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+      int key = 0x7fffffff;
+      if (n < NODES_G2P) {
         const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
         const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
-        float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
         if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
           const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
           const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
           v0 = vp[0];
           v1 = vp[1];
         }
-        s_vel[n][0] = v0;
+        if (v1.z == 0.0f) {
+          v0 = make_float4(0.f, 0.f, 0.f, 0.f);
+          v1 = v0;
+        } else {
+          key = ((abs(2 * x - (G2P_R - 1)) + abs(2 * y - (G2P_R - 1)) + abs(2 * z - (G2P_R - 1))) << 8) | n;
+        }
+      }
+      int kmin = key;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) kmin = min(kmin, __shfl_xor(kmin, off, 64));
+      if (key == kmin && key != 0x7fffffff) s_refv[tid >> 6] = v0;
+      if ((tid & 63) == 0) s_refkey[tid >> 6] = kmin;
+      __syncthreads();
+      int best = 0;
+#pragma unroll
+      for (int w = 1; w < WG / 64; w++) best = (s_refkey[w] < s_refkey[best]) ? w : best;
+      const float4 r = (s_refkey[best] != 0x7fffffff) ? s_refv[best] : make_float4(0.f, 0.f, 0.f, 0.f);
+      vref = make_float3(r.x, r.y, r.z);
+      if (n < NODES_G2P) {
+        s_vel[n][0] = make_float4(v0.x - r.x, v0.y - r.y, v0.z - r.z, v0.w);
         s_vel[n][1] = v1;
       }
     }
@@ -915,7 +971,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))
       // phase A
       int knew = -1;
       if (i < c1)
-        knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col,
+        knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, vref, s_mat, par, &s_rec, col,
                                                  MB ? s_vel2 : nullptr,
                                                  MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr);
       int hl = 0, hbase = 0;
@@ -1339,7 +1395,7 @@ __global__ __launch_bounds__(WG) void k_audit(DevPtrs d) {
     for (int a = 0; a < 3; a++) {
       const float x = f32(A[(F_VX + a) * cap + s]);
       fin = fin && isfinite(x);
-      v[a] = x;
+      v[a] = (double)x + (double)xv_res(A[(F_XV0 + a) * cap + s]);
     }
     for (int a = 0; a < 6; a++) {
       const float x = f32(A[(F_S0 + a) * cap + s]);

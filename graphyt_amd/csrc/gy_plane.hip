@@ -224,7 +224,7 @@ __device__ __noinline__ void plane_p2g_global(float *acc, uint8_t *brick_flag, c
 // objects; Extrapolation.cpp:288-367). The record is written when s_rec is given.
 template <bool UPDATE, int MASK, bool MB = false>
 __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I, int o0, int o1, double tl, int i,
-                                            int slot, const float4 (*s_vel)[2], const MatDev *s_mat, int par,
+                                            int slot, const float4 (*s_vel)[2], float3 vref, const MatDev *s_mat, int par,
                                             PlaneRec *s_rec, int col, const float4 (*s_vel2)[2] = nullptr,
                                             const int (*s_obj)[PG_N] = nullptr) {
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
@@ -244,6 +244,9 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
 
   if (UPDATE) {
     const int out = i;
+    uint32_t xv[3];  // position fraction | velocity residual (gy_internal.hpp)
+#pragma unroll
+    for (int a = 0; a < 3; a++) xv[a] = A[(F_XV0 + a) * cap + slot];
     bool bad = false;
     // ---- G2P (Extrapolation.cpp:127-163, 288-367 with shape_z = 1, gradShape_z = 0) ----
     const int b0 = stencil_base(q0, I.lpq, I.fbits, I.n[0] - 3, bad);
@@ -292,6 +295,9 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
         Ly[c] += gy * S0[c];
       }
     }
+    vg[0] += vref.x;
+    vg[1] += vref.y;
+    vg[2] += vref.z;
     float er[6];
     er[0] = Lx[0];
     er[1] = Ly[1];
@@ -308,17 +314,28 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
       if (bct) {
 #pragma unroll
         for (int k = 0; k < 4; k++) bv[k] = d.pbc_val[4 * (long long)pid + k];
+        const float v0[3] = {v[0], v[1], v[2]};
         particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 4 (Solver.cpp:219-223)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (v[a] != v0[a]) xv[a] &= 0xffffu;  // a prescribed velocity has no residual
       }
     }
     // ---- particle update (TimeIntegration.cpp:11-117): x and y only ----
     const float dt = I.dt;
     rho += rho_dot * dt;
     if (!tag_hold(tag)) {
+      int qq[2] = {q0, q1};
 #pragma unroll
-      for (int a = 0; a < 2; a++) v[a] = v[a] + (ap[a] - I.damping * v[a]) * dt;
-      q0 += __float2int_rn(vg[0] * I.qscale);
-      q1 += __float2int_rn(vg[1] * I.qscale);
+      for (int a = 0; a < 2; a++) {
+        float vr = xv_res(xv[a]);
+        uint32_t lo = xv_lo(xv[a]);
+        vel_add(v[a], vr, (ap[a] - I.damping * v[a]) * dt);
+        advance_q(qq[a], lo, vg[a] * I.qscale);
+        xv[a] = xv_pack(lo, vr);
+      }
+      q0 = qq[0];
+      q1 = qq[1];
     }
     float e[6];
 #pragma unroll
@@ -375,13 +392,21 @@ __device__ __forceinline__ int plane_update(const DevPtrs &d, const InstDesc &I,
     if (q1 > I.wrapq[1]) q1 = I.oneq + q1 % I.wrapq[1];
     if (q1 < I.oneq) q1 += I.wrapq[1] - I.oneq;
     if (MASK & KM_PBC) {
-      if (bct) particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+      if (bct) {
+        const float v0[3] = {v[0], v[1], v[2]};
+        particle_bc(bct, bv, (float)tl, v, ap, vg, s, mass);  // phase 5 (Solver.cpp:249-253)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (v[a] != v0[a]) xv[a] &= 0xffffu;
+      }
     }
 
     // ---- store the new state at its sorted position ----
     B[F_QX * cap + out] = (uint32_t)q0;
     B[F_QY * cap + out] = (uint32_t)q1;
     B[F_QZ * cap + out] = A[F_QZ * cap + slot];
+#pragma unroll
+    for (int a = 0; a < 3; a++) B[(F_XV0 + a) * cap + out] = xv[a];
 #pragma unroll
     for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
 #pragma unroll
@@ -494,6 +519,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ int s_off[TILE3 + 1];
   __shared__ MatDev s_mat[MAXMAT];
   __shared__ int s_tile;
+  __shared__ int s_refkey[WG / 64];
+  __shared__ float4 s_refv[WG / 64];
   const int tid = threadIdx.x;
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
@@ -509,6 +536,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const InstDesc I = d.inst[ii];
     const int lt = t - I.tile_base;
     const int o0 = (lt % I.nt[0]) * PT, o1 = (lt / I.nt[0]) * PT;
+    float3 vref = make_float3(0.f, 0.f, 0.f);  // staged node velocities are relative to vref
 
     for (int n = tid; n < 7 * PACC_N; n += WG) (&s_acc[0][0])[n] = 0.0;
     if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
@@ -516,15 +544,42 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if constexpr (MB) {
       for (int n = tid; n < PG_N; n += WG) plane_stage_body_vel(d, I, o0, o1, n, s_vel, s_vel2, s_obj);
     } else if (UPDATE) {
-      for (int n = tid; n < PG_N; n += WG) {
-        const int g0 = o0 + n % PG_R, g1 = o1 + n / PG_R;
-        float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+      // Node velocities staged relative to the tile's reference velocity (the massive node
+      // nearest the region centre), nodes without mass as zero: the 9 weights of a plane
+      // stencil sum to 1 and their gradients to 0, so the FP32 G2P sums round velocity
+      // differences, not full velocities (as in the 3D kernel).
+      static_assert(PG_N <= WG, "one staged node per thread");
+      const int n = tid;
+      float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+      int key = 0x7fffffff;
+      if (n < PG_N) {
+        const int x = n % PG_R, y = n / PG_R;
+        const int g0 = o0 + x, g1 = o1 + y;
         if (g0 < I.n[0] && g1 < I.n[1]) {
           const float4 *vp = reinterpret_cast<const float4 *>(d.vel + (I.node_base + g0 + (long long)I.n[0] * g1) * NODE_W);
           v0 = vp[0];
           v1 = vp[1];
         }
-        s_vel[n][0] = v0;
+        if (v1.z == 0.0f) {
+          v0 = make_float4(0.f, 0.f, 0.f, 0.f);
+          v1 = v0;
+        } else {
+          key = ((abs(2 * x - (PG_R - 1)) + abs(2 * y - (PG_R - 1))) << 8) | n;
+        }
+      }
+      int kmin = key;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) kmin = min(kmin, __shfl_xor(kmin, off, 64));
+      if (key == kmin && key != 0x7fffffff) s_refv[tid >> 6] = v0;
+      if ((tid & 63) == 0) s_refkey[tid >> 6] = kmin;
+      __syncthreads();
+      int best = 0;
+#pragma unroll
+      for (int w = 1; w < WG / 64; w++) best = (s_refkey[w] < s_refkey[best]) ? w : best;
+      const float4 r = (s_refkey[best] != 0x7fffffff) ? s_refv[best] : make_float4(0.f, 0.f, 0.f, 0.f);
+      vref = make_float3(r.x, r.y, r.z);
+      if (n < PG_N) {
+        s_vel[n][0] = make_float4(v0.x - r.x, v0.y - r.y, v0.z - r.z, v0.w);
         s_vel[n][1] = v1;
       }
     }
@@ -540,7 +595,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const int col = 16 * (j % P) + j / P;
       int knew = -1;
       if (i < c1)
-        knew = plane_update<UPDATE, MASK, MB>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, s_mat, par, &s_rec, col,
+        knew = plane_update<UPDATE, MASK, MB>(d, I, o0, o1, tl, i, d.tmp[i], s_vel, vref, s_mat, par, &s_rec, col,
                                               MB ? s_vel2 : nullptr,
                                               MB ? reinterpret_cast<const int (*)[PG_N]>(s_obj) : nullptr);
       int hl = 0, hbase = 0;

@@ -28,9 +28,11 @@ def test_each_instance_matches_its_own_oracle_run(oracle_mod):
         assert o.dt == pytest.approx(b.dt_of(i), rel=1e-15)
         o.run(150)
         ref, got = o.get(FIELDS), b.download(i, FIELDS)
-        assert rel(got["pos"], ref["pos"]) <= 1e-6, i
-        assert rel(got["vel"], ref["vel"]) <= 1e-3, i
-        assert rel(got["stress"], ref["stress"]) <= 1e-3, i
+        r = {k: rel(got[k], ref[k]) for k in ("pos", "vel", "stress")}
+        print(f"instance {i} ({sc.n} particles): " + ", ".join(f"{k} {v:.2e}" for k, v in r.items()), flush=True)
+        assert r["pos"] <= 1e-6, i
+        assert r["vel"] <= 1e-4, i
+        assert r["stress"] <= 1e-4, i
     b.close()
 
 

@@ -60,6 +60,7 @@ def two_blocks(steps, mu=0.0, shear=0.0):
 def test_rigid_platen_200_steps(oracle_mod):
     a, b, audit = run_both(oracle_mod, platen_scene(200), 200)
     for k, tol in (("pos", 1e-6), ("vel", 1e-4), ("stress", 1e-4)):
+        print(f"{k}: rel L2 {rel(a[k], b[k]):.2e} (tol {tol:.0e})", flush=True)
         assert rel(a[k], b[k]) <= tol, k
     assert audit["nonfinite"] == 0
 
@@ -67,12 +68,14 @@ def test_rigid_platen_200_steps(oracle_mod):
 def test_two_deformable_bodies_in_contact(oracle_mod):
     a, b, _ = run_both(oracle_mod, two_blocks(100), 100)
     for k, tol in (("pos", 1e-6), ("vel", 1e-4), ("stress", 1e-4)):
+        print(f"{k}: rel L2 {rel(a[k], b[k]):.2e} (tol {tol:.0e})", flush=True)
         assert rel(a[k], b[k]) <= tol, k
 
 
 def test_contact_with_friction(oracle_mod):
     a, b, _ = run_both(oracle_mod, two_blocks(80, mu=0.3, shear=0.2), 80)
     for k, tol in (("pos", 1e-6), ("vel", 1e-4), ("stress", 1e-4)):
+        print(f"{k}: rel L2 {rel(a[k], b[k]):.2e} (tol {tol:.0e})", flush=True)
         assert rel(a[k], b[k]) <= tol, k
 
 
@@ -90,8 +93,8 @@ def test_contact_graph_replay_matches_eager():
 def test_brazilian_disk_with_rigid_platens(oracle_mod):
     """The Grady-Kipp disk between two rigid platens (three objects), scaled to h = 0.004
     (19,960 particles), 300 steps: contact, rigid override and the FP64 eig3 path.
-    Velocity and stress follow the oracle to 3e-4: at this resolution the platens touch the
-    disk on a single node row, where the slot claims decide the contact response."""
+    Velocity, stress and s_max follow the oracle to 1e-4 (reached on MI355X: 2.0e-5,
+    1.7e-5, 5.1e-5)."""
     sc = W.gk_disk(h=0.004, steps=300)
     g = runtime.Context(sc)
     g.run(300)
@@ -101,5 +104,6 @@ def test_brazilian_disk_with_rigid_platens(oracle_mod):
     o.run(300)
     b = o.get(["pos", "vel", "stress", "smax"])
     assert (b["smax"] > 0).any()
-    for k, tol in (("pos", 1e-6), ("vel", 3e-4), ("stress", 3e-4), ("smax", 1e-3)):
+    for k, tol in (("pos", 1e-6), ("vel", 1e-4), ("stress", 1e-4), ("smax", 1e-4)):
+        print(f"{k}: rel L2 {rel(a[k], b[k]):.2e} (tol {tol:.0e})", flush=True)
         assert rel(a[k], b[k]) <= tol, k

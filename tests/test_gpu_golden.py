@@ -31,22 +31,22 @@ TOL = {
     "elastic": ELASTIC,
     "newtonian": ELASTIC,
     "turbulent": ELASTIC,
-    "perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
-    "dp_cohesive": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-3, "acc_plastic_strain": 1e-3},
+    "perfect_plastic": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "acc_plastic_strain": 1e-4},
+    "dp_cohesive": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "acc_plastic_strain": 1e-4},
     "grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
     "bcs": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "rigid_platen": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "two_blocks_mu": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "plane_dam_break": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "strain": 1e-4, "density": 1e-6},
     "plane_elastic": ELASTIC,
-    "plane_perfect_plastic": {"pos": 1e-6, "vel": 1e-3, "stress": 1e-3, "acc_plastic_strain": 1e-2},
+    "plane_perfect_plastic": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "acc_plastic_strain": 1e-4},
     "plane_turbulent": ELASTIC,
     "plane_grady_kipp": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "damage": 1e-4, "smax": 1e-4},
     # Rigid contact in a plane: against the oracle binning and claiming body slots as the GPU
     # does (scenes.CLAIM_ORDER); test_gpu_claim_order_distance bounds the distance to the
     # reference's own fixture.
     "plane_rigid_platen": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
-    "plane_brazilian": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "acc_plastic_strain": 1e-3, "damage": 1e-3,
+    "plane_brazilian": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "acc_plastic_strain": 1e-4, "damage": 1e-4,
                         "smax": 1e-4},
     "plane_two_blocks": {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4},
     "thermal": dict(ELASTIC, temperature=1e-5),
@@ -110,14 +110,28 @@ def test_gpu_matches_reference_fixture(name, oracle_mod):
 
 def test_gpu_unguarded_apex_nan_share():
     """c = 0, psi = 0, no guard: the reference's apex return is NaN (0*inf) for yielding
-    particles in tension. The mean stress at step 0 is rounding noise, so which particles
-    go NaN depends on FP64 vs FP32 rounding; the share must be comparable, and the finite
-    particles agree."""
+    particles whose trial mean stress is tensile (ConstitutiveModels.cpp:364-365). One step
+    from rest, most particles sit where the velocity field is uniform (m*g*dt/m), so their
+    strain rate, and hence the sign of their trial mean stress, is rounding noise: the
+    reference, which sums full node velocities in FP64, turns about half of them NaN; this
+    path sums node velocities relative to a tile reference, where a uniform field cancels
+    exactly, and far fewer go NaN. What does not depend on the noise: the unguarded map
+    gives NaN here too, the guarded map gives none on the same scene, and the particles
+    finite on both sides agree."""
     out, fx = gpu_run("dp_unguarded")
     na, nb = np.isnan(out["stress"]).any(axis=1), np.isnan(fx["stress"]).any(axis=1)
-    assert nb.sum() > 0 and 0.5 * nb.sum() < na.sum() < 2.0 * nb.sum()
+    print(f"NaN particles: GPU {na.sum()}, reference {nb.sum()} of {na.size}")
+    assert nb.sum() > 0 and na.sum() > 0
     both = ~na & ~nb
     assert rel(out["pos"][both], fx["pos"][both]) <= 1e-6
+    sc, steps, _ = scenes.build("dp_unguarded")
+    for m in sc.mats:
+        m.dp_apex_guard = 1
+    g = runtime.Context(sc)
+    g.run(steps)
+    r = g.download(["stress"])
+    g.close()
+    assert np.isfinite(r["stress"]).all()
 
 
 def test_gpu_plane_z_is_not_advanced():
