@@ -11,9 +11,10 @@ gravity -9.81, dt = 0.1*h/sqrt(K/rho). Synthetic data generated in-process.
 Prints ONE JSON line on stdout (everything else goes to stderr). `value` is whole-job
 particle-steps/s with the state resident in HBM; `roofline` is the dominant kernel's
 algorithmic bytes per launch over its average duration, timed with HIP events on the
-context's own stream during the timed region; `cpu_baseline` times the reference's
-iterate_CPU (compiled from its sources into oracle/_ref) on a 200-step sample of the same
-workload on this host's cores.
+context's own stream during the timed region; `cpu_baseline` times the FP64 OpenMP
+restatement of the reference's iterate_CPU (its loops, schedules and sort cadence, with
+the guarded apex: the reference's own map turns this column NaN) on a 60-step sample of
+the same workload on this host's cores, kind "port".
 """
 from __future__ import annotations
 
@@ -61,12 +62,25 @@ def log(*a):
 
 
 def cpu_baseline(sample_steps: int, warm: int, workload: str = "dp_column") -> dict:
-    """Time the CPU reference on the same workload (run in a child process)."""
+    """Time the CPU reference step on the same workload (run in a child process).
+
+    dp_column: the FP64 OpenMP restatement of iterate_CPU (oracle/cpu_ref.cpp: the
+    reference's loops, schedule clauses and every-10-steps sort) with the guarded apex,
+    kind "port". The compiled reference cannot run this workload: with c = 0 and psi = 0
+    its apex return is 0*inf (ConstitutiveModels.cpp:364-365) and the column's stresses
+    turn NaN within a few steps, after which the gather does almost no work.
+    plane_dam_break: the compiled reference itself (kind "reference") where it is built.
+    A sample whose final state holds a non-finite value is not reported as a rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle
     from graphyt_amd import workloads as W
-    kind = "reference" if oracle.available("reference") else "restatement"
-    sc = plane_scene() if workload == "plane_dam_break" else W.dp_column()
+    if workload == "plane_dam_break":
+        kind = "reference" if oracle.available("reference") else "restatement"
+        sc = plane_scene()
+    else:
+        kind = "restatement"
+        sc = W.dp_column()
     sim = oracle.CpuSim(sc, kind)
     for s in range(warm):
         sim.iterate(s * sim.dt, s)
@@ -74,12 +88,34 @@ def cpu_baseline(sample_steps: int, warm: int, workload: str = "dp_column") -> d
     for s in range(warm, warm + sample_steps):
         sim.iterate(s * sim.dt, s)
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": sc.n * sample_steps / dt, "unit": "particle-steps/s", "cores": cores,
-            "kind": "reference" if kind == "reference" else "port",
-            "sample": f"{workload} {sc.n:,} particles, steps {warm}..{warm + sample_steps - 1} "
-                      f"({sample_steps} steps incl. the every-10-steps sort), FP64, "
-                      f"OMP_NUM_THREADS={cores}, {dt:.1f} s"}
+    st = sim.get(["pos", "vel", "stress"])
+    sim.close()
+    finite = all(bool(np.isfinite(st[k]).all()) for k in st)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    out = {"value": sc.n * sample_steps / dt if finite else None, "unit": "particle-steps/s", "cores": threads,
+           "kind": "reference" if kind == "reference" else "port",
+           "sample": f"{workload} {sc.n:,} particles, steps {warm}..{warm + sample_steps - 1} "
+                     f"({sample_steps} steps incl. the every-10-steps sort), FP64, {dt:.1f} s"
+                     + ("" if kind == "reference" else ", guarded apex (the reference's map is NaN here)"),
+           "nproc": os.cpu_count(), "OMP_NUM_THREADS": threads,
+           "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES"),
+           "final_state_finite": finite}
+    return out
+
+
+def run_cpu_baseline(cpu_steps: int, workload: str = "dp_column"):
+    """The CPU leg in a child process with the threads pinned as BASELINE.md asks
+    (OMP_PROC_BIND=close, OMP_PLACES=cores unless set)."""
+    env = dict(os.environ)
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", workload,
+                            "--cpu-steps", str(cpu_steps)], capture_output=True, text=True, timeout=900, env=env)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # the GPU figure stands on its own
+        log("cpu baseline failed:", e)
+        return None
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
@@ -255,15 +291,7 @@ def bench_plane(args) -> int:
     avg_s = total_ms / launches / 1e3
     bpl = B_ALG_PLANE_G2P2G * sc.n
     achieved = bpl / avg_s / 1e9
-    cpu = None
-    if not args.no_cpu:
-        try:
-            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload",
-                                "plane_dam_break", "--cpu-steps", str(min(args.cpu_steps, 30))], capture_output=True,
-                               text=True, timeout=900)
-            cpu = json.loads(r.stdout.strip().splitlines()[-1])
-        except Exception as e:
-            log("cpu baseline failed:", e)
+    cpu = None if args.no_cpu else run_cpu_baseline(min(args.cpu_steps, 30), "plane_dam_break")
     out = {
         "metric": "particle-steps/s (plane GIMP, 0.94M pts)", "value": round(best, 1), "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * sc.n / best, 4),
@@ -327,7 +355,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--profile-json", default="")
@@ -403,15 +431,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch}
 
-    cpu = None
-    if not args.no_cpu:
-        try:
-            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only",
-                                "--cpu-steps", str(args.cpu_steps)], capture_output=True, text=True,
-                               timeout=900)
-            cpu = json.loads(r.stdout.strip().splitlines()[-1])
-        except Exception as e:  # the GPU figure stands on its own
-            log("cpu baseline failed:", e)
+    cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_steps)
 
     best = max(value, value_graph)
     out = {

@@ -24,7 +24,9 @@ extern "C" {
 /* ---- status codes --------------------------------------------------------------- */
 #define GY_OK              0
 #define GY_ERR_INVALID     1   /* bad argument or inconsistent sizes                   */
-#define GY_ERR_UNSUPPORTED 2   /* a feature outside the implemented path (2D, model 5) */
+#define GY_ERR_UNSUPPORTED 2   /* a request outside the implemented path: more than 16
+                                  materials, psep > cellsize, a model id outside 1..5,
+                                  mixed 3D and plane instances, > 2^31 particles      */
 #define GY_ERR_HIP         3   /* a HIP runtime call failed                            */
 #define GY_ERR_RANGE       4   /* a particle's GIMP stencil left the grid              */
 #define GY_ERR_STATE       5   /* call order violated (e.g. iterate before upload)     */
@@ -61,7 +63,8 @@ typedef struct gy_grid {
 /* One MaterialModels entry (MaterialModels.hpp:5-48, defaults MaterialModels.cpp:7-25). */
 typedef struct gy_material {
   int32_t model;          /* 1 elastic, 2 Newtonian, 3 perfect plastic, 4 Drucker-Prager,
-                             5 turbulent liquid (its flag lives in the damage field) */
+                             5 turbulent liquid (its flag lives in the damage field);
+                             all five run in 3D and plane contexts */
   int32_t is_rigid;
   int32_t is_damage;
   int32_t damage_model;   /* 1 = Grady-Kipp */
@@ -77,7 +80,8 @@ typedef struct gy_material {
   double  crack_speed;    /* Cg */
   double  damage_m;
   double  damage_k;
-  double  thermal_conductivity;   /* carried for completeness; thermal path not simulated */
+  double  thermal_conductivity;   /* heat transfer (Extrapolation.cpp:281-283, 359-362) runs
+                                     when any uploaded particle temperature is nonzero */
   double  specific_heat;
   double  turb_stress;
   /* Drucker-Prager apex guard (not in the reference): when nonzero, the products
@@ -98,7 +102,9 @@ typedef struct gy_params {
   double  cfl;
   double  psep;
   double  dt;
-  int32_t is3D;           /* must be 1 */
+  int32_t is3D;           /* Parameters::is3D: 1 = the 3D step; 0 = the plane step per unit
+                             depth (L[2] = 0, one node layer, 9-node stencil, plane model
+                             branches, Extrapolation.cpp:90-124) */
   int32_t reserved;
 } gy_params;
 
@@ -178,7 +184,9 @@ int  gy_iterate_n(gy_ctx *ctx, double t0, int32_t step0, int32_t nsteps);
 /* Wait for the context stream; reports a deferred device-side error (GY_ERR_RANGE). */
 int  gy_synchronize(gy_ctx *ctx);
 
-/* Download particle fields (FP32 device state widened to FP64) in the caller's order. */
+/* Download particle fields in the caller's order: FP32 device state widened to FP64;
+ * positions are 48-bit fixed point (h/2^(F+16)) and velocities FP32 plus a bfloat16
+ * residual, both exact in FP64. */
 int  gy_download(gy_ctx *ctx, gy_particles *host);
 
 /* Download the node fields of the last step (Nodes::getMassArr / getVelArr /

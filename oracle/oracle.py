@@ -6,6 +6,11 @@ the graphyt_amd product never does.
   * kind="restatement": oracle/libcpu_ref.so, the FP64 restatement (cpu_ref.cpp)
   * kind="reference":   oracle/_ref/libgraphyt_ref.so, the reference's own sources compiled
                         in place by oracle/Makefile (only where the reference checkout exists)
+  * kind="reference_gpu": oracle/_ref/libgraphyt_ref_gpu.so, the same reference build with
+                        its Solver::iterate_GPU slot filled by the Graphyt-side binding of
+                        the HIP step (graphyt_amd/integration/graphyt_iterate_gpu.cpp):
+                        `iterate_gpu` / `synchronise` run the product through the
+                        reference's own classes, the way a Graphyt script would
 """
 from __future__ import annotations
 
@@ -26,6 +31,7 @@ from graphyt_amd.abi import GyGrid, GyMaterial, GyParams, GyParticles, ParticleA
 LIBS = {
     "restatement": (os.environ.get("CREF_LIB", os.path.join(HERE, "libcpu_ref.so")), "cref_"),
     "reference": (os.path.join(HERE, "_ref", "libgraphyt_ref.so"), "rref_"),
+    "reference_gpu": (os.path.join(HERE, "_ref", "libgraphyt_ref_gpu.so"), "rref_"),
 }
 _loaded = {}
 
@@ -64,6 +70,13 @@ def _lib(kind: str):
     fn("get_nodes").argtypes = [P] + [C.POINTER(C.c_double)] * 5
     fn("eig3").argtypes = [C.POINTER(C.c_double)] * 3
     fn("destroy").argtypes = [P]
+    if kind == "reference_gpu":
+        for name in ("iterate_gpu", "synchronise", "iterate_gpu_n"):
+            fn(name).restype = C.c_int32
+        fn("iterate_gpu").argtypes = [P, C.c_double, C.c_int32]
+        fn("iterate_gpu_n").argtypes = [P, C.c_double, C.c_int32, C.c_int32]
+        fn("synchronise").argtypes = [P]
+        fn("gpu_error").restype = C.c_char_p
     if kind == "restatement":
         lib.cref_constitutive.argtypes = [C.POINTER(GyMaterial)] + [C.POINTER(C.c_double)] * 5 + [C.c_double]
         lib.cref_gimp.restype = C.c_double
@@ -92,7 +105,7 @@ class CpuSim:
         re-bins every step by stencil-base cell with a cell's particles in object order,
         as the GPU path bins and claims body slots."""
         self.kind = kind
-        if kind == "reference":
+        if kind in ("reference", "reference_gpu"):
             # The reference's dynamic OpenMP chunk, P/(16*threads), is evaluated inside the
             # team (Solver.cpp:120-249): below 16 particles per thread it is 0 and libgomp
             # never advances the loop, so the step would hang.
@@ -143,6 +156,24 @@ class CpuSim:
         if t0 is None:
             t0 = step0 * self.dt
         self._f("iterate_n")(self.h, t0, step0, nsteps)
+
+    # -- the reference's Solver through its iterate_GPU slot (kind "reference_gpu") --
+    def _gpu_rc(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._f("gpu_error")().decode())
+
+    def iterate_gpu(self, t: float, step: int) -> None:
+        """solver.iterate_GPU(t, step, parameters)"""
+        self._gpu_rc(self._f("iterate_gpu")(self.h, t, step))
+
+    def run_gpu(self, nsteps: int, step0: int = 0, t0: float | None = None) -> None:
+        if t0 is None:
+            t0 = step0 * self.dt
+        self._gpu_rc(self._f("iterate_gpu_n")(self.h, t0, step0, nsteps))
+
+    def synchronise(self) -> None:
+        """solver.Synchronise(): device state back into MaterialPoints and Nodes"""
+        self._gpu_rc(self._f("synchronise")(self.h))
 
     def get(self, fields=OUT_FIELDS) -> dict:
         pa = ParticleArrays.empty(self.scene.n, fields)

@@ -6,6 +6,12 @@
 // (cpu_ref.cpp) and bench.py may time it as the CPU baseline. It exposes the same
 // signatures as cpu_ref.cpp with the `rref_` prefix.
 //
+// Built a second time with -DGY_REF_GPU into oracle/_ref/libgraphyt_ref_gpu.so, together
+// with the Graphyt-side binding graphyt_amd/integration/graphyt_iterate_gpu.cpp (the
+// bodies of Solver::iterate_GPU / Synchronise / sync_GPU) and libgraphyt_amd.so: then the
+// reference's own Solver runs the MI355X step through its iterate_GPU slot
+// (rref_iterate_gpu, rref_synchronise), which the GPU tests drive like a Graphyt script.
+//
 // Scenario set-up follows the reference's Python usage (README.md:180-294):
 // Nodes(L, h); MaterialPoints(nodes); psep; add(i, x, y, z) per particle; numParticles;
 // per-particle setters; Parameters(tmax, is3D) + setters; Solver(...); optional setDt.
@@ -19,6 +25,11 @@
 
 #include "Solver.hpp"
 #include "../include/graphyt_amd.h"
+#ifdef GY_REF_GPU
+#include <exception>
+#include <string>
+extern "C" void graphyt_gpu_release(const Solver *s);
+#endif
 
 namespace {
 
@@ -210,8 +221,48 @@ void rref_eig3(const double *A9, double *V9, double *d3) {
   for (int i = 0; i < 9; i++) V9[i] = V[i / 3][i % 3];
 }
 
+#ifdef GY_REF_GPU
+static std::string g_gpu_error;
+
+// solver.iterate_GPU(t, step, parameters) and solver.Synchronise() of the reference's
+// Solver with the MI355X binding; 0 or -1 (message in rref_gpu_error) on an exception.
+int32_t rref_iterate_gpu(void *h, double t, int32_t step) {
+  Ref *r = (Ref *)h;
+  try {
+    r->sol->iterate_GPU(t, step, r->par);
+  } catch (const std::exception &e) {
+    g_gpu_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int32_t rref_iterate_gpu_n(void *h, double t0, int32_t step0, int32_t n) {
+  Ref *r = (Ref *)h;
+  for (int i = 0; i < n; i++)
+    if (rref_iterate_gpu(h, t0 + i * r->sol->dt, step0 + i) != 0) return -1;
+  return 0;
+}
+
+int32_t rref_synchronise(void *h) {
+  Ref *r = (Ref *)h;
+  try {
+    r->sol->Synchronise();
+  } catch (const std::exception &e) {
+    g_gpu_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+const char *rref_gpu_error() { return g_gpu_error.c_str(); }
+#endif
+
 void rref_destroy(void *h) {
   Ref *r = (Ref *)h;
+#ifdef GY_REF_GPU
+  graphyt_gpu_release(r->sol);
+#endif
   delete r->sol;
   delete r->par;
   delete r->mp;

@@ -43,6 +43,9 @@ def main(steps: int = 1000) -> int:
     out.update(steps=steps, particles=sc.n, max_abs_pos_diff=float(np.abs(a["pos"] - b["pos"]).max()),
                nonfinite_gpu=int(np.isnan(a["stress"]).sum()), nonfinite_oracle=int(np.isnan(b["stress"]).sum()))
     print(json.dumps(out), flush=True)
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", f"full_parity_{steps}.json"), "w") as f:
+        json.dump(out, f)
     return 0
 
 
