@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -952,6 +953,102 @@ int gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *cou
   (void)hipStreamDestroy(tmp.stream);
   tmp.stream = nullptr;
   return rc;
+}
+
+}  // extern "C"
+
+// ---- gy_counting_sort_noalloc: the counting sort without any allocation -------------
+// Its scratch is one file-scope device array, `scratch`: every GPU that runs the entry
+// point holds its own copy until the process ends. Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): the device address
+// comes from hipGetSymbolAddress, once per device, and only that address is used.
+namespace {
+constexpr long long NA_MAX_N = 1LL << 20;      // keys per call
+constexpr long long NA_MAX_KEYS = 1LL << 20;   // key space per call
+constexpr long long na_align(long long w) { return (w + 63) / 64 * 64; }  // 256-byte segments
+constexpr long long NA_KEYS_PAD = (NA_MAX_KEYS + TILE3 - 1) / TILE3 * TILE3;
+constexpr int NA_WG = 256;
+constexpr long long NA_SCAN_BLOCK = 4096;  // keys per scan block (scan_items_per_block)
+constexpr long long NA_BLOCKS = (NA_KEYS_PAD + NA_SCAN_BLOCK - 1) / NA_SCAN_BLOCK;
+// layout in 4-byte words: rank[n] | tmp[n] | count[keys] | offsets[keys + 1] | blocksum |
+// blocktiles | active tiles | Ctrl
+constexpr long long NA_RANK = 0;
+constexpr long long NA_TMP = NA_RANK + na_align(NA_MAX_N);
+constexpr long long NA_COUNT = NA_TMP + na_align(NA_MAX_N);
+constexpr long long NA_OFF = NA_COUNT + na_align(NA_KEYS_PAD);
+constexpr long long NA_BSUM = NA_OFF + na_align(NA_KEYS_PAD + 1);
+constexpr long long NA_BTILES = NA_BSUM + na_align(NA_BLOCKS);
+constexpr long long NA_ACTIVE = NA_BTILES + na_align(NA_BLOCKS);
+constexpr long long NA_CTRL = NA_ACTIVE + na_align(NA_KEYS_PAD / TILE3 + 1);
+constexpr long long NA_WORDS = NA_CTRL + na_align((long long)(sizeof(Ctrl) + 3) / 4);
+
+__device__ __align__(256) uint32_t scratch[NA_WORDS];
+
+std::mutex g_na_mu;
+void *g_na_ptr[256] = {};  // device address of `scratch`, per device ordinal
+
+// zero the histogram and the control block (no memset node: callers may capture this)
+__global__ void k_na_clear(uint32_t *base, long long nkeys_pad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nkeys_pad) base[NA_COUNT + i] = 0u;
+  if (i < (long long)((sizeof(Ctrl) + 3) / 4)) base[NA_CTRL + i] = 0u;
+}
+
+// the caller's counts and offsets from the scratch offsets
+__global__ void k_na_out(const uint32_t *base, long long nkeys, int32_t *counts, int32_t *offsets) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nkeys) return;
+  const int32_t *off = reinterpret_cast<const int32_t *>(base + NA_OFF);
+  if (offsets) offsets[k] = off[k];
+  if (counts) counts[k] = off[k + 1] - off[k];
+}
+}  // namespace
+
+extern "C" {
+
+int gy_counting_sort_noalloc(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *counts, int32_t *offsets,
+                             int32_t *order, void *stream) {
+  if (!keys || n <= 0 || nkeys <= 0) return GY_ERR_INVALID;
+  if (n > NA_MAX_N || nkeys > NA_MAX_KEYS) return GY_ERR_INVALID;  // beyond the scratch
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256) return GY_ERR_HIP;
+  uint32_t *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+      void *p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GY_ERR_HIP;
+      g_na_ptr[dev] = p;
+    }
+    base = static_cast<uint32_t *>(g_na_ptr[dev]);
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevPtrs d{};
+  d.np = (int)n;
+  d.nkeys = (int)((nkeys + TILE3 - 1) / TILE3 * TILE3);
+  d.nblocks_scan = (int)((d.nkeys + scan_items_per_block() - 1) / scan_items_per_block());
+  if (d.nblocks_scan > NA_BLOCKS) return GY_ERR_INVALID;
+  d.key[0] = const_cast<int *>(keys);
+  d.rank = reinterpret_cast<int *>(base + NA_RANK);
+  d.tmp = reinterpret_cast<int *>(base + NA_TMP);
+  d.count = reinterpret_cast<int *>(base + NA_COUNT);
+  d.offsets = reinterpret_cast<int *>(base + NA_OFF);
+  d.blocksum = reinterpret_cast<int *>(base + NA_BSUM);
+  d.blocktiles = reinterpret_cast<int *>(base + NA_BTILES);
+  d.active = reinterpret_cast<int *>(base + NA_ACTIVE);
+  d.ctrl = reinterpret_cast<Ctrl *>(base + NA_CTRL);
+  const long long clear = std::max<long long>(d.nkeys, (long long)((sizeof(Ctrl) + 3) / 4));
+  hipLaunchKernelGGL(k_na_clear, dim3((unsigned)((clear + NA_WG - 1) / NA_WG)), dim3(NA_WG), 0, st, base,
+                     (long long)d.nkeys);
+  launch_hist(d, st);
+  launch_scan_partial(d, st);
+  launch_scan_final(d, st);
+  launch_place(d, st);
+  if (order) launch_stable_order(d, order, st);
+  if (counts || offsets)
+    hipLaunchKernelGGL(k_na_out, dim3((unsigned)((nkeys + NA_WG - 1) / NA_WG)), dim3(NA_WG), 0, st, base,
+                       (long long)nkeys, counts, offsets);
+  return hipGetLastError() == hipSuccess ? GY_OK : GY_ERR_HIP;
 }
 
 int gy_autotune(gy_ctx *c, int32_t trial_steps, int64_t *best_wgs, double *best_ms) {

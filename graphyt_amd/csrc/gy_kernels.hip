@@ -100,10 +100,14 @@ __device__ __forceinline__ void block_scan_n(int (&v)[N], int (&total)[N]) {
   __syncthreads();
 }
 
+// Histogram of caller-supplied keys (the standalone counting sorts): keys outside
+// [0, nkeys) are skipped here, in k_place and in k_stable_order.
 __global__ __launch_bounds__(WG) void k_hist(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
   if (s >= d.np) return;
-  d.rank[s] = atomicAdd(&d.count[d.key[0][s]], 1);
+  const int k = d.key[0][s];
+  if (k < 0 || k >= d.nkeys) return;
+  d.rank[s] = atomicAdd(&d.count[k], 1);
 }
 
 // Scan blocks cover SCAN_BLOCK keys = SCAN_BLOCK / TILE3 whole tiles; four consecutive
@@ -253,6 +257,7 @@ __global__ __launch_bounds__(WG) void k_place(DevPtrs d) {
   const int s = blockIdx.x * WG + threadIdx.x;
   if (s >= d.np) return;
   const int k = (d.ctrl->parity ? d.key[1] : d.key[0])[s];
+  if (k < 0 || k >= d.nkeys) return;
   d.tmp[d.offsets[k] + d.rank[s]] = s;
 }
 
@@ -270,6 +275,7 @@ __device__ __forceinline__ int stable_pos(const DevPtrs &d, const int *keys, int
 __global__ __launch_bounds__(WG) void k_stable_order(DevPtrs d, int *order) {
   const int i = blockIdx.x * WG + threadIdx.x;
   if (i >= d.np) return;
+  if (i >= d.offsets[d.nkeys]) return;  // entries of skipped (out-of-range) keys
   const int slot = d.tmp[i];
   order[stable_pos(d, (d.ctrl->parity ? d.key[1] : d.key[0]), i, slot)] = slot;
 }

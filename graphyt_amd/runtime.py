@@ -36,6 +36,8 @@ SIGNATURES = {
     "gy_audit": (C.c_int, [_P, C.POINTER(GyAudit)]),
     "gy_debug_binning": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I64P, _I64P]),
     "gy_counting_sort": (C.c_int, [_I32P, C.c_int64, C.c_int64, _I32P, _I32P, _I32P]),
+    # device pointers (keys, counts, offsets, order) and a hipStream_t
+    "gy_counting_sort_noalloc": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, _P, _P]),
     "gy_create_batch": (C.c_int, [_PP, C.c_int32, C.POINTER(GyGrid), C.POINTER(GyMaterial), _I32P,
                                   C.POINTER(GyParams)]),
     "gy_instance_count": (C.c_int, [_P]),

@@ -209,6 +209,20 @@ int  gy_debug_binning(gy_ctx *ctx, int32_t *keys, int32_t *counts, int32_t *offs
 int  gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys,
                       int32_t *counts, int32_t *offsets, int32_t *order);
 
+/* Zero-allocation variant of gy_counting_sort. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory — code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. `keys` (n), `counts`, `offsets`
+ * (nkeys each) and `order` (n) are DEVICE pointers (counts, offsets, order may be NULL);
+ * `stream` is a hipStream_t (NULL: the null stream). The call only enqueues work on
+ * `stream`: no allocation, no free, no synchronisation; results are ready when the stream
+ * is. Keys outside [0, nkeys) are skipped (neither counted nor placed). Limits:
+ * n <= 1,048,576 and nkeys <= 1,048,576 (GY_ERR_INVALID beyond them). Its scratch is a
+ * library-owned device array of 16.8 MB that each GPU it runs on holds until the process
+ * ends; every call on a GPU reuses that GPU's array, so calls that overlap on one GPU
+ * (two streams, two threads) race on it and must be serialised by the caller. */
+int  gy_counting_sort_noalloc(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *counts,
+                              int32_t *offsets, int32_t *order, void *stream);
+
 /* ---- batched ensembles ----------------------------------------------------------
  * No counterpart in the reference, which runs one Solver per problem: `ninst`
  * independent problems advanced together by every gy_iterate / gy_iterate_n call, in
