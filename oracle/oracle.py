@@ -188,6 +188,13 @@ class CpuSim:
                              _dp(out["acc_b1"]), _dp(out["force"]))
         return out
 
+    def serialise(self, step: int, name: str) -> None:
+        """MaterialPoints::Serialise + Nodes::Serialise of the reference (its own CSV writers)
+        into output/ of the current directory (kinds "reference" and "reference_gpu")."""
+        f = self._f("serialise")
+        f.argtypes = [C.c_void_p, C.c_int32, C.c_char_p]
+        f(self.h, int(step), name.encode())
+
     def close(self):
         if self.h:
             self._f("destroy")(self.h)

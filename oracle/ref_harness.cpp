@@ -213,6 +213,16 @@ void rref_get_nodes(void *h, double *mass, double *vel, double *vb1, double *ab1
   if (force) std::memcpy(force, nd->force, 3 * N * sizeof(double));
 }
 
+// MaterialPoints::Serialise and Nodes::Serialise (MaterialPoints.cpp:341-364,
+// Nodes.cpp:165-177): output/<name>_mpts_<step>.csv and output/<name>_nds_<step>.csv under
+// the current directory, particles in the reference's current (sorted) order.
+void rref_serialise(void *h, int32_t step, const char *name) {
+  Ref *r = (Ref *)h;
+  std::vector<char> buf(name, name + std::strlen(name) + 1);
+  r->mp->Serialise(step, buf.data());
+  r->nodes->Serialise(step, buf.data());
+}
+
 // The reference's own eigen-decomposition (eig3.cpp:258-267), for pinning the oracle's.
 void rref_eig3(const double *A9, double *V9, double *d3) {
   double A[3][3], V[3][3];
