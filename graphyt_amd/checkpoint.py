@@ -13,10 +13,12 @@ step)` / `Solver.restore(path)`, written in C++) and the C-ABI contexts of
 Particle arrays use the names of `abi.PARTICLE_FIELDS` (pos, pos0, vel, stress, strain,
 plastic_strain, acc_plastic_strain, density, mass, volume, damage, smax, material,
 obj_id, hold, bc_type, bc_val, cell_id, temperature, temperature_grad), node BCs are
-node_bc_type / node_bc_val, and the grid is grid_L, grid_h, psep. FP32 device values and
-fixed-point positions survive the FP64 round trip exactly, so a restored run continues
-from the same state; it is re-binned on upload, which reorders the FP32 sums only
-(agreement to rounding).
+node_bc_type / node_bc_val, and the grid is grid_L, grid_h, psep. `device_snapshot`
+(uint8) holds the device state between two steps (gy_snapshot_save): restored into a
+context of the same setup it gives back that state bit for bit, binning, node records,
+tile partials and the previous step's time (read by the sine particle BCs) included.
+Files without it restore from the particle arrays alone: re-binned on upload, which
+reorders the FP32 sums (agreement to rounding).
 """
 from __future__ import annotations
 
@@ -78,7 +80,8 @@ def read(path: str) -> tuple:
 
 def save(ctx, scene, path: str, t: float, step: int) -> None:
     """Checkpoint a single-instance runtime.Context running `scene` (the scene supplies the
-    grid, materials and node BCs; the particle state comes from the device)."""
+    grid, materials and node BCs; the particle arrays and the device snapshot come from the
+    device)."""
     from .abi import PARTICLE_FIELDS
     out = ctx.download(PARTICLE_NAMES)
     arrays = {k: np.asarray(out[k]).reshape(-1) for k in PARTICLE_NAMES}
@@ -89,6 +92,7 @@ def save(ctx, scene, path: str, t: float, step: int) -> None:
     arrays["grid_h"] = np.asarray([scene.h], np.float64)
     arrays["psep"] = np.asarray([scene.psep], np.float64)
     assert set(PARTICLE_NAMES) <= set(PARTICLE_FIELDS)
+    arrays["device_snapshot"] = ctx.snapshot()
     write(path, arrays, t, step, ctx.dt)
 
 
@@ -115,3 +119,16 @@ def load(path: str, scene):
         scene.node_bc_val = arrays["node_bc_val"].reshape(-1, 4)
     scene.dt = dt
     return scene, t, step
+
+
+def restore(path: str, scene, graph: bool = True):
+    """A runtime.Context continuing a checkpoint of `scene`'s setup: the checkpoint's
+    particle arrays uploaded and, when the file holds one, its device snapshot loaded (the
+    exact device state of the checkpointed step). Returns (context, t, step)."""
+    from .runtime import Context
+    scene, t, step = load(path, scene)
+    arrays, _, _, _ = read(path)
+    ctx = Context(scene, graph=graph)
+    if "device_snapshot" in arrays:
+        ctx.load_snapshot(arrays["device_snapshot"])
+    return ctx, t, step

@@ -1051,6 +1051,195 @@ int gy_counting_sort_noalloc(const int32_t *keys, int64_t n, int64_t nkeys, int3
   return hipGetLastError() == hipSuccess ? GY_OK : GY_ERR_HIP;
 }
 
+}  // extern "C"
+
+// ---- snapshot of the device state between two steps ---------------------------------
+// Everything the next step reads that earlier steps wrote: the current particle buffer,
+// keys, ranks, the slot list, counts and offsets, scan partials and the tile list, the
+// node records (direct-scatter sums, solved velocities), BCs, brick flags and list, the
+// control block, the times, the contact and heat buffers, and the tile partials of the
+// tiles that wrote them this step (the grid update reads no other tile's partials: their
+// epoch stamps are stale). Loading it into a context set up and uploaded the same way
+// puts that state back bit for bit.
+namespace {
+struct SnapHead {
+  char magic[8];
+  uint32_t version, ninst;
+  int64_t np, cap, nnodes, ntiles, nkeys, nbricks, nmats;
+  int32_t parity, acc_valid, multibody, thermal, kmask, host_step, ncore, nhalo;
+};
+struct Seg {
+  void *dev;
+  size_t bytes;
+};
+const char kSnapMagic[8] = {'G', 'Y', 'S', 'N', 'A', 'P', '\0', '\1'};
+
+std::vector<Seg> snap_segments(gy_ctx *c, int parity) {
+  DevPtrs &d = c->d;
+  const size_t cap = (size_t)d.cap, nn = (size_t)c->nnodes, nt = (size_t)d.ntiles, nk = (size_t)d.nkeys,
+               nb = (size_t)d.nbricks, ni = (size_t)d.ninst, ns = (size_t)d.nblocks_scan;
+  std::vector<Seg> v = {{d.pbuf[parity], 4 * (size_t)NF * cap}, {d.key[0], 4 * cap}, {d.key[1], 4 * cap},
+                        {d.rank, 4 * cap}, {d.tmp, 4 * cap}, {d.pbc_val, 16 * cap}, {d.count, 4 * nk},
+                        {d.offsets, 4 * (nk + 1)}, {d.blocksum, 4 * ns}, {d.blocktiles, 4 * ns},
+                        {d.active, 4 * (nt + 1)}, {d.tile_epoch, 4 * nt}, {d.halo_epoch, 4 * nt},
+                        {d.acc, 4 * (size_t)NODE_W * nn}, {d.vel, 4 * (size_t)NODE_W * nn}, {d.bc_type, nn},
+                        {d.bc_val, 8 * nn}, {d.brick_flag, nb}, {d.brick_list, 4 * nb}, {d.ctrl, sizeof(Ctrl)},
+                        {d.t_last, 8 * ni}, {d.t_cur, 8 * ni}};
+  if (d.multibody) {
+    v.push_back({d.acc2, 4 * (size_t)MB_ACC2_W * nn});
+    v.push_back({d.nrm, 24 * nn});
+    v.push_back({d.claim, 16 * nn});
+    v.push_back({d.velb, 4 * (size_t)MB_VELB_W * nn});
+    v.push_back({d.cellobj, 16 * nk});
+    v.push_back({d.twotile, 4 * nt});
+  }
+  if (d.thermal) v.push_back({d.therm, 8 * nn});
+  return v;
+}
+
+constexpr size_t CORE_BYTES = 4 * (size_t)CORE_W + 8 * (P2G_NODES / 64);  // core partials + node mask
+constexpr size_t HALO_BYTES = 4 * (size_t)PARTIAL_W;
+
+// The tiles whose core (halo-shell) partials are current; the snapshot's size.
+int snap_layout(gy_ctx *c, Ctrl &h, std::vector<int> &core, std::vector<int> &halo, size_t &bytes) {
+  DevPtrs &d = c->d;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  std::vector<int> te((size_t)d.ntiles), he((size_t)d.ntiles);
+  HIPCHECK(c, hipMemcpy(te.data(), d.tile_epoch, te.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(he.data(), d.halo_epoch, he.size() * 4, hipMemcpyDeviceToHost));
+  core.clear();
+  halo.clear();
+  for (int t = 0; t < d.ntiles; t++) {
+    if (te[t] == h.epoch) core.push_back(t);
+    if (he[t] == h.epoch) halo.push_back(t);
+  }
+  bytes = sizeof(SnapHead) + c->inst.size() * sizeof(InstDesc) + c->mats.size() * sizeof(MatDev);
+  for (const auto &H : c->ih) bytes += sizeof(double) + H.carry_plastic.size() + H.carry_damage.size();
+  for (const Seg &g : snap_segments(c, h.parity)) bytes += g.bytes;
+  bytes += core.size() * (4 + CORE_BYTES) + halo.size() * (4 + HALO_BYTES);
+  return GY_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int gy_snapshot_size(gy_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  Ctrl h{};
+  std::vector<int> core, halo;
+  size_t n = 0;
+  int rc = snap_layout(c, h, core, halo, n);
+  if (rc) return rc;
+  *bytes = (int64_t)n;
+  return GY_OK;
+}
+
+int gy_snapshot_save(gy_ctx *c, void *buf, int64_t bytes) {
+  if (!c || !buf) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  Ctrl h{};
+  std::vector<int> core, halo;
+  size_t n = 0;
+  int rc = snap_layout(c, h, core, halo, n);
+  if (rc) return rc;
+  if (bytes != (int64_t)n) return fail(c, GY_ERR_INVALID, "snapshot buffer has %lld bytes, the state needs %zu", (long long)bytes, n);
+  DevPtrs &d = c->d;
+  char *p = static_cast<char *>(buf);
+  SnapHead H{};
+  std::memcpy(H.magic, kSnapMagic, 8);
+  H.version = 1;
+  H.ninst = (uint32_t)c->inst.size();
+  H.np = d.np; H.cap = d.cap; H.nnodes = c->nnodes; H.ntiles = d.ntiles; H.nkeys = d.nkeys; H.nbricks = d.nbricks;
+  H.nmats = (int64_t)c->mats.size();
+  H.parity = h.parity; H.acc_valid = c->acc_valid ? 1 : 0; H.multibody = d.multibody; H.thermal = d.thermal;
+  H.kmask = d.kmask; H.host_step = c->host_step; H.ncore = (int32_t)core.size(); H.nhalo = (int32_t)halo.size();
+  std::memcpy(p, &H, sizeof H); p += sizeof H;
+  std::memcpy(p, c->inst.data(), c->inst.size() * sizeof(InstDesc)); p += c->inst.size() * sizeof(InstDesc);
+  std::memcpy(p, c->mats.data(), c->mats.size() * sizeof(MatDev)); p += c->mats.size() * sizeof(MatDev);
+  for (const auto &I : c->ih) {
+    std::memcpy(p, &I.dt, sizeof(double)); p += sizeof(double);
+    std::memcpy(p, I.carry_plastic.data(), I.carry_plastic.size()); p += I.carry_plastic.size();
+    std::memcpy(p, I.carry_damage.data(), I.carry_damage.size()); p += I.carry_damage.size();
+  }
+  for (const Seg &g : snap_segments(c, h.parity)) {
+    HIPCHECK(c, hipMemcpy(p, g.dev, g.bytes, hipMemcpyDeviceToHost));
+    p += g.bytes;
+  }
+  std::memcpy(p, core.data(), core.size() * 4); p += core.size() * 4;
+  for (int t : core) {
+    HIPCHECK(c, hipMemcpyAsync(p, d.pcore + (size_t)t * CORE_W, 4 * (size_t)CORE_W, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(p + 4 * (size_t)CORE_W, d.nodemask + (size_t)t * (P2G_NODES / 64), 8 * (P2G_NODES / 64),
+                               hipMemcpyDeviceToHost, c->stream));
+    p += CORE_BYTES;
+  }
+  std::memcpy(p, halo.data(), halo.size() * 4); p += halo.size() * 4;
+  for (int t : halo) {
+    HIPCHECK(c, hipMemcpyAsync(p, d.partial + (size_t)t * PARTIAL_W, HALO_BYTES, hipMemcpyDeviceToHost, c->stream));
+    p += HALO_BYTES;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return (size_t)(p - static_cast<char *>(buf)) == n ? GY_OK : fail(c, GY_ERR_STATE, "snapshot layout mismatch");
+}
+
+int gy_snapshot_load(gy_ctx *c, const void *buf, int64_t bytes) {
+  if (!c || !buf) return GY_ERR_INVALID;
+  if (!c->uploaded) return fail(c, GY_ERR_STATE, "upload the particles of the same setup before loading a snapshot");
+  DevPtrs &d = c->d;
+  const char *p = static_cast<const char *>(buf), *end = p + bytes;
+  SnapHead H;
+  if (bytes < (int64_t)sizeof H) return fail(c, GY_ERR_INVALID, "snapshot too short");
+  std::memcpy(&H, p, sizeof H); p += sizeof H;
+  if (std::memcmp(H.magic, kSnapMagic, 8) != 0 || H.version != 1) return fail(c, GY_ERR_INVALID, "not a snapshot of this format");
+  if (H.ninst != c->inst.size() || H.np != d.np || H.cap != d.cap || H.nnodes != c->nnodes || H.ntiles != d.ntiles ||
+      H.nkeys != d.nkeys || H.nbricks != d.nbricks || H.nmats != (int64_t)c->mats.size())
+    return fail(c, GY_ERR_INVALID, "the snapshot was taken of a different setup (grid, instances, materials or particle count)");
+  if (H.multibody != d.multibody || H.thermal != d.thermal)
+    return fail(c, GY_ERR_INVALID, "the snapshot's contact / heat-transfer mode differs from the uploaded particles'");
+  if (H.parity != 0 && H.parity != 1) return fail(c, GY_ERR_INVALID, "corrupt snapshot");
+  size_t need = c->inst.size() * sizeof(InstDesc) + c->mats.size() * sizeof(MatDev);
+  for (const auto &I : c->ih) need += sizeof(double) + I.carry_plastic.size() + I.carry_damage.size();
+  for (const Seg &g : snap_segments(c, H.parity)) need += g.bytes;
+  need += (size_t)H.ncore * (4 + CORE_BYTES) + (size_t)H.nhalo * (4 + HALO_BYTES);
+  if (H.ncore < 0 || H.nhalo < 0 || (size_t)(end - p) != need) return fail(c, GY_ERR_INVALID, "snapshot size mismatch");
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  std::memcpy(c->inst.data(), p, c->inst.size() * sizeof(InstDesc)); p += c->inst.size() * sizeof(InstDesc);
+  std::memcpy(c->mats.data(), p, c->mats.size() * sizeof(MatDev)); p += c->mats.size() * sizeof(MatDev);
+  for (auto &I : c->ih) {
+    std::memcpy(&I.dt, p, sizeof(double)); p += sizeof(double);
+    std::memcpy(I.carry_plastic.data(), p, I.carry_plastic.size()); p += I.carry_plastic.size();
+    std::memcpy(I.carry_damage.data(), p, I.carry_damage.size()); p += I.carry_damage.size();
+  }
+  for (const Seg &g : snap_segments(c, H.parity)) {
+    HIPCHECK(c, hipMemcpy(g.dev, p, g.bytes, hipMemcpyHostToDevice));
+    p += g.bytes;
+  }
+  std::vector<int> core((size_t)H.ncore), halo((size_t)H.nhalo);
+  std::memcpy(core.data(), p, core.size() * 4); p += core.size() * 4;
+  for (int t : core) {
+    if (t < 0 || t >= d.ntiles) return fail(c, GY_ERR_INVALID, "corrupt snapshot tile list");
+    HIPCHECK(c, hipMemcpyAsync(d.pcore + (size_t)t * CORE_W, p, 4 * (size_t)CORE_W, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d.nodemask + (size_t)t * (P2G_NODES / 64), p + 4 * (size_t)CORE_W, 8 * (P2G_NODES / 64),
+                               hipMemcpyHostToDevice, c->stream));
+    p += CORE_BYTES;
+  }
+  std::memcpy(halo.data(), p, halo.size() * 4); p += halo.size() * 4;
+  for (int t : halo) {
+    if (t < 0 || t >= d.ntiles) return fail(c, GY_ERR_INVALID, "corrupt snapshot tile list");
+    HIPCHECK(c, hipMemcpyAsync(d.partial + (size_t)t * PARTIAL_W, p, HALO_BYTES, hipMemcpyHostToDevice, c->stream));
+    p += HALO_BYTES;
+  }
+  int rc = push_inst(c);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  d.kmask = H.kmask;
+  c->acc_valid = H.acc_valid != 0;
+  c->host_step = H.host_step;
+  return GY_OK;
+}
+
 int gy_autotune(gy_ctx *c, int32_t trial_steps, int64_t *best_wgs, double *best_ms) {
   if (!c) return GY_ERR_INVALID;
   if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");

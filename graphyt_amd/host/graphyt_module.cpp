@@ -491,6 +491,15 @@ class Solver {
         {"node_bc_val", 0, nds->BCVAL.size(), nds->BCVAL.data()},
         {"grid_L", 0, nds->L.size(), nds->L.data()}, {"grid_h", 0, 1, &nds->cellsize},
         {"psep", 0, 1, &P.psep}};
+    // the device state of the last step (gy_snapshot_save), when there is a device context
+    std::vector<uint8_t> snap;
+    if (ctx) {
+      int64_t nb = 0;
+      check(ctx, gy_snapshot_size(ctx, &nb), "gy_snapshot_size");
+      snap.resize((size_t)nb);
+      check(ctx, gy_snapshot_save(ctx, snap.data(), nb), "gy_snapshot_save");
+      arrs.push_back({"device_snapshot", 2, snap.size(), snap.data()});
+    }
     FILE *fp = std::fopen(path.c_str(), "wb");
     if (!fp) throw std::runtime_error("cannot open " + path);
     const size_t esize[3] = {8, 4, 1};
@@ -511,7 +520,10 @@ class Solver {
   }
 
   // Restore a checkpoint written for the same setup (grid, particle count, materials):
-  // the host arrays are replaced and uploaded before the next step. Returns (t, step).
+  // the host arrays are replaced; with the checkpoint's device snapshot the device context
+  // is set up from them and given back the checkpointed device state bit for bit (binning,
+  // node records, the previous step's time), else they are uploaded before the next step.
+  // Returns (t, step).
   std::pair<double, int64_t> restore(const std::string &path) {
     FILE *fp = std::fopen(path.c_str(), "rb");
     if (!fp) throw std::runtime_error("cannot open " + path);
@@ -539,6 +551,7 @@ class Solver {
     take(td, 16);
     take(&step, 8);
     Synchronise();  // nothing the device holds is newer than what the checkpoint replaces
+    std::vector<uint8_t> snap;
     auto &P = *mpts;
     const size_t n = P.size();
     const size_t esize[3] = {8, 4, 1};
@@ -583,7 +596,10 @@ class Solver {
       else if (name == "cell_id") fill(P.cellID, 1, n);
       else if (name == "node_bc_type") fill(nds->BCTYPE, 1, (size_t)nds->numNodes);
       else if (name == "node_bc_val") fill(nds->BCVAL, 0, 4 * (size_t)nds->numNodes);
-      else if (name == "grid_h") {
+      else if (name == "device_snapshot") {
+        if (code != 2) throw std::runtime_error(path + ": bad device snapshot");
+        snap.assign(reinterpret_cast<const uint8_t *>(src), reinterpret_cast<const uint8_t *>(src) + nbytes);
+      } else if (name == "grid_h") {
         double h;
         std::memcpy(&h, src, 8);
         if (code != 0 || count != 1 || h != nds->cellsize) throw std::runtime_error(path + ": grid spacing differs");
@@ -596,6 +612,10 @@ class Solver {
     P.host_dirty = true;
     P.device_newer = false;
     nds->bc_dirty = true;
+    if (!snap.empty()) {
+      prepare(prm);  // context, node BCs and the restored arrays on the device
+      check(ctx, gy_snapshot_load(ctx, snap.data(), (int64_t)snap.size()), "gy_snapshot_load");
+    }
     return {td[0], step};
   }
 

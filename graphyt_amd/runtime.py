@@ -36,6 +36,9 @@ SIGNATURES = {
     "gy_audit": (C.c_int, [_P, C.POINTER(GyAudit)]),
     "gy_debug_binning": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I64P, _I64P]),
     "gy_counting_sort": (C.c_int, [_I32P, C.c_int64, C.c_int64, _I32P, _I32P, _I32P]),
+    "gy_snapshot_size": (C.c_int, [_P, _I64P]),
+    "gy_snapshot_save": (C.c_int, [_P, _P, C.c_int64]),
+    "gy_snapshot_load": (C.c_int, [_P, _P, C.c_int64]),
     # device pointers (keys, counts, offsets, order) and a hipStream_t
     "gy_counting_sort_noalloc": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P, _P, _P]),
     "gy_create_batch": (C.c_int, [_PP, C.c_int32, C.POINTER(GyGrid), C.POINTER(GyMaterial), _I32P,
@@ -162,6 +165,19 @@ class Context:
 
     def autotune(self, trial_steps: int = 20):
         return autotune(self, trial_steps)
+
+    def snapshot(self) -> np.ndarray:
+        """The device state between two steps (gy_snapshot_save) as bytes."""
+        n = C.c_int64()
+        self._check(self.lib.gy_snapshot_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        self._check(self.lib.gy_snapshot_save(self.h, buf.ctypes.data_as(_P), n.value))
+        return buf
+
+    def load_snapshot(self, buf) -> None:
+        """Put a snapshot of a context with the same setup back (gy_snapshot_load)."""
+        buf = np.ascontiguousarray(buf, np.uint8)
+        self._check(self.lib.gy_snapshot_load(self.h, buf.ctypes.data_as(_P), buf.size))
 
     def audit(self) -> dict:
         a = GyAudit()

@@ -223,6 +223,19 @@ int  gy_counting_sort(const int32_t *keys, int64_t n, int64_t nkeys,
 int  gy_counting_sort_noalloc(const int32_t *keys, int64_t n, int64_t nkeys, int32_t *counts,
                               int32_t *offsets, int32_t *order, void *stream);
 
+/* Snapshot of the device state between two steps (no counterpart in the reference, which
+ * has no checkpoint): every buffer the next step reads that earlier steps wrote — the
+ * current particle buffer, the binning, the node records, the tile partials of this step,
+ * the control block and the previous step's time (read by the sine particle BCs), the
+ * contact and heat buffers. gy_snapshot_load puts it back bit for bit into a context that
+ * was created and uploaded with the same setup (grid, materials, parameters, particle
+ * count and modes); gy_iterate then continues from it. The step itself is reproducible to
+ * rounding only (the order of its atomic adds varies between runs), so a continuation
+ * follows the original run as closely as two runs of it follow each other. */
+int  gy_snapshot_size(gy_ctx *ctx, int64_t *bytes);
+int  gy_snapshot_save(gy_ctx *ctx, void *buf, int64_t bytes);
+int  gy_snapshot_load(gy_ctx *ctx, const void *buf, int64_t bytes);
+
 /* ---- batched ensembles ----------------------------------------------------------
  * No counterpart in the reference, which runs one Solver per problem: `ninst`
  * independent problems advanced together by every gy_iterate / gy_iterate_n call, in
