@@ -79,6 +79,14 @@ class GyAudit(C.Structure):
     ]
 
 
+class GyTuning(C.Structure):
+    """gy_tuning: launch shapes chosen by the autotuner."""
+    _fields_ = [
+        ("persist_wgs", C.c_int64), ("fused_waves", C.c_int32), ("grid_wgs_factor", C.c_int32),
+        ("ms_per_step", C.c_double), ("from_cache", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 # field name -> (components, numpy dtype)
 PARTICLE_FIELDS = {
     "pos": (3, np.float64), "pos0": (3, np.float64), "vel": (3, np.float64),

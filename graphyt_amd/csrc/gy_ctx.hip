@@ -317,6 +317,8 @@ int alloc_grid(gy_ctx *c) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
     d.persist_wgs = 8 * ((cus * 3 + 7) / 8);  // three resident 256-thread workgroups per CU
+    d.fused_waves = 3;
+    d.grid_factor = 2;
   }
   if ((rc = dalloc(c, &d.inst, c->inst.size()))) return rc;
   if ((rc = dalloc(c, &d.mats, c->mats.size()))) return rc;
@@ -1240,67 +1242,188 @@ int gy_snapshot_load(gy_ctx *c, const void *buf, int64_t bytes) {
   return GY_OK;
 }
 
-int gy_autotune(gy_ctx *c, int32_t trial_steps, int64_t *best_wgs, double *best_ms) {
-  if (!c) return GY_ERR_INVALID;
+}  // extern "C"
+
+// ---- autotuner ----------------------------------------------------------------------
+// Launch shapes tried per workload: persistent particle workgroups per CU (1, 2, 3, 4, 6),
+// the fused kernel's build (3 waves per SIMD at 168 VGPRs, or 2 at 256) and the grid
+// kernel's workgroups per persistent workgroup (1, 2, 4). The cell-tile edge is not among
+// them: at edge 4 the FP64 P2G region of a tile (8^3 nodes x 7 sums) already takes
+// 28.7 KB of the 160 KB LDS of a CU, and edge 6 would take 89.6 KB (one workgroup per
+// CU). Each trial starts from the same device state (a snapshot), and the state is put
+// back afterwards. The winner can be kept in a cache file, one line per workload shape.
+namespace {
+std::string tune_key(gy_ctx *c) {
+  hipDeviceProp_t pr{};
+  std::string arch = "unknown";
+  int cus = 0;
+  if (hipGetDeviceProperties(&pr, c->device) == hipSuccess) {
+    arch = pr.gcnArchName;
+    cus = pr.multiProcessorCount;
+  }
+  for (char &ch : arch)
+    if (ch == ' ' || ch == '|') ch = '_';
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "v1|%s|cu%d|inst%zu|np%d|nodes%lld|tiles%d|kmask%d|mb%d|th%d|plane%d", arch.c_str(), cus,
+                c->inst.size(), c->d.np, c->nnodes, c->d.ntiles, c->d.kmask, c->d.multibody, c->d.thermal, c->plane ? 1 : 0);
+  return buf;
+}
+
+bool tune_lookup(const char *path, const std::string &key, gy_tuning &t) {
+  FILE *fp = std::fopen(path, "r");
+  if (!fp) return false;
+  char line[1024];
+  bool found = false;
+  while (std::fgets(line, sizeof line, fp)) {
+    char k[600];
+    long long wgs = 0;
+    int waves = 0, factor = 0;
+    double ms = 0;
+    if (std::sscanf(line, "%599s %lld %d %d %lf", k, &wgs, &waves, &factor, &ms) == 5 && key == k) {
+      t.persist_wgs = wgs;
+      t.fused_waves = waves;
+      t.grid_wgs_factor = factor;
+      t.ms_per_step = ms;
+      found = true;  // the last matching line wins
+    }
+  }
+  std::fclose(fp);
+  return found;
+}
+
+int tune_store(const char *path, const std::string &key, const gy_tuning &t) {
+  std::vector<std::string> keep;
+  if (FILE *fp = std::fopen(path, "r")) {
+    char line[1024];
+    while (std::fgets(line, sizeof line, fp)) {
+      char k[600];
+      if (std::sscanf(line, "%599s", k) == 1 && key != k) keep.push_back(line);
+    }
+    std::fclose(fp);
+  }
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE *fp = std::fopen(tmp.c_str(), "w");
+  if (!fp) return GY_ERR_INVALID;
+  for (const auto &l : keep) std::fputs(l.c_str(), fp);
+  std::fprintf(fp, "%s %lld %d %d %.6f\n", key.c_str(), (long long)t.persist_wgs, t.fused_waves, t.grid_wgs_factor,
+               t.ms_per_step);
+  if (std::fclose(fp) != 0 || std::rename(tmp.c_str(), path) != 0) return GY_ERR_INVALID;
+  return GY_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int gy_get_tuning(const gy_ctx *c, gy_tuning *t) {
+  if (!c || !t) return GY_ERR_INVALID;
+  *t = gy_tuning();
+  t->persist_wgs = c->d.persist_wgs;
+  t->fused_waves = c->d.fused_waves;
+  t->grid_wgs_factor = c->d.grid_factor;
+  return GY_OK;
+}
+
+int gy_set_tuning(gy_ctx *c, const gy_tuning *t) {
+  if (!c || !t) return GY_ERR_INVALID;
+  if (t->persist_wgs < 8 || t->persist_wgs > (1 << 20) || (t->fused_waves != 2 && t->fused_waves != 3) ||
+      (t->grid_wgs_factor != 1 && t->grid_wgs_factor != 2 && t->grid_wgs_factor != 4))
+    return fail(c, GY_ERR_INVALID, "tuning out of range (workgroups >= 8, waves 2 or 3, grid factor 1, 2 or 4)");
+  c->d.persist_wgs = (int)(8 * ((t->persist_wgs + 7) / 8));
+  c->d.fused_waves = t->fused_waves;
+  c->d.grid_factor = t->grid_wgs_factor;
+  drop_graph(c);
+  return GY_OK;
+}
+
+int gy_autotune_shapes(gy_ctx *c, int32_t trial_steps, const char *cache_path, gy_tuning *out) {
+  if (!c || !out) return GY_ERR_INVALID;
   if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
   if (trial_steps < 1 || trial_steps > 1000) return fail(c, GY_ERR_INVALID, "trial_steps must be in [1, 1000]");
   int rc = gy_synchronize(c);
   if (rc) return rc;
-  DevPtrs &d = c->d;
+  const std::string key = tune_key(c);
+  gy_tuning t = gy_tuning();
+  if (cache_path && *cache_path && tune_lookup(cache_path, key, t)) {
+    if ((rc = gy_set_tuning(c, &t))) return rc;
+    t.from_cache = 1;
+    *out = t;
+    return GY_OK;
+  }
   Ctrl h{};
-  HIPCHECK(c, hipMemcpy(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
-  // snapshot of the particle state and per-instance step times; trials run on the live
-  // buffers and the snapshot is put back afterwards (re-binned by the next step)
-  const size_t words = (size_t)NF * d.cap;
-  uint32_t *snap = nullptr;
-  HIPCHECK(c, hipMalloc((void **)&snap, words * 4));
-  std::vector<double> tl(c->inst.size());
-  HIPCHECK(c, hipMemcpy(snap, d.pbuf[h.parity], words * 4, hipMemcpyDeviceToDevice));
-  HIPCHECK(c, hipMemcpy(tl.data(), d.t_last, tl.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(&h, c->d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  int64_t nbytes = 0;
+  if ((rc = gy_snapshot_size(c, &nbytes))) return rc;
+  std::vector<uint8_t> snap((size_t)nbytes);
+  if ((rc = gy_snapshot_save(c, snap.data(), nbytes))) return rc;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) cus = 256;
-  const int per_cu[] = {1, 2, 3, 4, 6};
   const bool saved_prof = c->profile;
   c->profile = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  int best = d.persist_wgs;
-  double best_t = 1e300;
-  auto restore = [&]() -> int {
-    HIPCHECK(c, hipMemcpyAsync(d.pbuf[0], snap, words * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(d.t_last, tl.data(), tl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    launch_set_ctrl(d, h.step, 0.0, 1, c->stream);
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    c->acc_valid = false;
-    return GY_OK;
-  };
-  for (int k : per_cu) {
-    const int wgs = 8 * ((cus * k + 7) / 8);
-    d.persist_wgs = wgs;
-    drop_graph(c);
-    if ((rc = restore())) break;
-    if ((rc = iterate_impl(c, 0.0, h.step, 1))) break;  // binning + first step (untimed)
+  // one trial: from the snapshot, an untimed step, then trial_steps timed steps
+  auto trial = [&](const gy_tuning &cand, double &ms_out) -> int {
+    int r = gy_set_tuning(c, &cand);
+    if (r) return r;
+    if ((r = gy_snapshot_load(c, snap.data(), nbytes))) return r;
+    if ((r = iterate_impl(c, 0.0, h.step, 1))) return r;
     (void)hipEventRecord(e0, c->stream);
-    if ((rc = iterate_impl(c, 0.0, h.step + 1, trial_steps))) break;
+    if ((r = iterate_impl(c, 0.0, h.step + 1, trial_steps))) return r;
     (void)hipEventRecord(e1, c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = fail(c, GY_ERR_HIP, "autotune trial failed"); break; }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, GY_ERR_HIP, "autotune trial failed");
+    if ((r = check_status(c))) return r;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    const double per = ms / trial_steps;
-    if (per < best_t) { best_t = per; best = wgs; }
+    ms_out = ms / trial_steps;
+    return GY_OK;
+  };
+  gy_tuning best = gy_tuning();
+  best.ms_per_step = 1e300;
+  const int per_cu[] = {1, 2, 3, 4, 6};
+  const bool single_body = !c->d.multibody && !c->plane;  // the 2-wave build exists for the 3D single-body kernel
+  for (int waves : {3, 2}) {
+    if (waves == 2 && !single_body) break;
+    for (int k : per_cu) {
+      gy_tuning cand = gy_tuning();
+      cand.persist_wgs = 8 * ((cus * k + 7) / 8);
+      cand.fused_waves = waves;
+      cand.grid_wgs_factor = 2;
+      double ms = 0;
+      if ((rc = trial(cand, ms))) break;
+      if (ms < best.ms_per_step) { best = cand; best.ms_per_step = ms; }
+    }
+    if (rc) break;
   }
+  if (!rc)
+    for (int f : {1, 4}) {
+      gy_tuning cand = best;
+      cand.grid_wgs_factor = f;
+      double ms = 0;
+      if ((rc = trial(cand, ms))) break;
+      if (ms < best.ms_per_step) { best = cand; best.ms_per_step = ms; }
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->profile = saved_prof;
-  d.persist_wgs = best;
-  drop_graph(c);
-  const int rc2 = restore();
-  (void)hipFree(snap);
+  const int rc2 = gy_set_tuning(c, rc ? &t : &best);  // on failure: back to the defaults
+  const int rc3 = gy_snapshot_load(c, snap.data(), nbytes);
   if (rc) return rc;
   if (rc2) return rc2;
-  if (best_wgs) *best_wgs = best;
-  if (best_ms) *best_ms = best_t;
+  if (rc3) return rc3;
+  best.from_cache = 0;
+  if (cache_path && *cache_path && tune_store(cache_path, key, best) != GY_OK)
+    return fail(c, GY_ERR_INVALID, "cannot write the tuning cache %s", cache_path);
+  *out = best;
+  return GY_OK;
+}
+
+int gy_autotune(gy_ctx *c, int32_t trial_steps, int64_t *best_wgs, double *best_ms) {
+  gy_tuning t = gy_tuning();
+  const int rc = gy_autotune_shapes(c, trial_steps, nullptr, &t);
+  if (rc) return rc;
+  if (best_wgs) *best_wgs = t.persist_wgs;
+  if (best_ms) *best_ms = t.ms_per_step;
   return GY_OK;
 }
 

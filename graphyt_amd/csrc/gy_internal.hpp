@@ -188,6 +188,8 @@ struct DevPtrs {
   int nblocks_scan;
   int kmask;             // model-set bits of the fused kernel specialization
   int persist_wgs;       // workgroups of the persistent particle kernel (multiple of 8)
+  int fused_waves;       // build of the single-body fused kernel: 3 or 2 waves per SIMD
+  int grid_factor;       // grid-kernel workgroups per persistent particle workgroup (1, 2, 4)
   // two-body contact (more than one object or rigid materials); see the MB kernels
   int multibody;
   float *acc2;           // MB_ACC2_W per canonical node: body 1 and 2 (m, p, f, rigid mass)

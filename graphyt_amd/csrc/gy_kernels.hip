@@ -847,8 +847,10 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 // fields follow the re-binning and the claims (k_mb_p2g). Body 2's fields take LDS
 // beyond three workgroups per CU, so the contact variant runs two.
 
-template <bool UPDATE, int MASK, bool MB = false>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : 3))) void k_g2p2g(DevPtrs d) {
+// WAVES: resident waves per SIMD the build targets, 3 (168 VGPRs) or 2 (256 VGPRs, no
+// spills, fewer waves to hide latency); the autotuner picks one per workload.
+template <bool UPDATE, int MASK, bool MB = false, int WAVES = 3>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAVES))) void k_g2p2g(DevPtrs d) {
   static_assert(WG == 4 * TILE3, "four waves per tile");
   static_assert(!MB || UPDATE, "the contact variant moves particles");
   constexpr int NB2 = MB ? NODES_G2P : 1;
@@ -1467,21 +1469,27 @@ void launch_place(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_place, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
 // Smallest built model-set specialization covering the context's mask.
-template <bool U>
-void launch_fused(const DevPtrs &d, hipStream_t st) {
+template <bool U, int W>
+void launch_fused_w(const DevPtrs &d, hipStream_t st) {
   const int m = d.kmask;
   const dim3 g(d.persist_wgs), b(WG);
-  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP>), g, b, 0, st, d);
-  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC>), g, b, 0, st, d);
-  else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT>), g, b, 0, st, d);
-  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC | KM_GK>), g, b, 0, st, d);
-  else hipLaunchKernelGGL((k_g2p2g<U, KM_ALL>), g, b, 0, st, d);
+  if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP, false, W>), g, b, 0, st, d);
+  else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC, false, W>), g, b, 0, st, d);
+  else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT, false, W>), g, b, 0, st, d);
+  else if ((m & ~(KM_ELASTIC | KM_GK)) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC | KM_GK, false, W>), g, b, 0, st, d);
+  else hipLaunchKernelGGL((k_g2p2g<U, KM_ALL, false, W>), g, b, 0, st, d);
+}
+// the P2G of an upload runs once: only the stepping kernel has the 2-wave build
+template <bool U>
+void launch_fused(const DevPtrs &d, hipStream_t st) {
+  if (U && d.fused_waves == 2) launch_fused_w<U, 2>(d, st);
+  else launch_fused_w<U, 3>(d, st);
 }
 
 void launch_p2g_only(const DevPtrs &d, void *st) { launch_fused<false>(d, S(st)); }
 void launch_grid(const DevPtrs &d, void *st) {
-  // persistent: at most two workgroups per persistent particle workgroup
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  // persistent: grid_factor workgroups per persistent particle workgroup at most
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
   hipLaunchKernelGGL(k_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 void launch_brick_list(const DevPtrs &d, void *st) {
@@ -1495,7 +1503,7 @@ void launch_audit(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_audit_final, dim3(1), dim3(WG), 0, S(st), d, nb);
 }
 void launch_mb_grid(const DevPtrs &d, void *st) {
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
   hipLaunchKernelGGL(k_mb_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 void launch_mb_g2p2g(const DevPtrs &d, void *st) {

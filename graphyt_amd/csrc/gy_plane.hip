@@ -878,7 +878,7 @@ void launch_plane_mb_p2g(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_plane_mb_p2g, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
 void launch_plane_mb_grid(const DevPtrs &d, void *st) {
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
   hipLaunchKernelGGL(k_plane_mb_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
@@ -892,7 +892,7 @@ void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
   else hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ALL, true>), g, b, 0, S(st), d);
 }
 void launch_plane_grid(const DevPtrs &d, void *st) {
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), 2 * d.persist_wgs));
+  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
   hipLaunchKernelGGL(k_plane_grid, dim3(nb), dim3(WG), 0, S(st), d);
 }
 

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .abi import (GY_OK, GyAudit, GyGrid, GyMaterial, GyParams, GyParticles, ParticleArrays,
+from .abi import (GY_OK, GyAudit, GyTuning, GyGrid, GyMaterial, GyParams, GyParticles, ParticleArrays,
                   material_array)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libgraphyt_amd.so")
@@ -36,6 +36,9 @@ SIGNATURES = {
     "gy_audit": (C.c_int, [_P, C.POINTER(GyAudit)]),
     "gy_debug_binning": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I64P, _I64P]),
     "gy_counting_sort": (C.c_int, [_I32P, C.c_int64, C.c_int64, _I32P, _I32P, _I32P]),
+    "gy_autotune_shapes": (C.c_int, [_P, C.c_int32, C.c_char_p, C.POINTER(GyTuning)]),
+    "gy_get_tuning": (C.c_int, [_P, C.POINTER(GyTuning)]),
+    "gy_set_tuning": (C.c_int, [_P, C.POINTER(GyTuning)]),
     "gy_snapshot_size": (C.c_int, [_P, _I64P]),
     "gy_snapshot_save": (C.c_int, [_P, _P, C.c_int64]),
     "gy_snapshot_load": (C.c_int, [_P, _P, C.c_int64]),
@@ -101,6 +104,11 @@ class GyError(RuntimeError):
 
 def _i32(a):
     return np.ascontiguousarray(a, np.int32)
+
+
+def tuning_dict(t) -> dict:
+    return {"persist_wgs": t.persist_wgs, "fused_waves": t.fused_waves, "grid_wgs_factor": t.grid_wgs_factor,
+            "ms_per_step": t.ms_per_step, "from_cache": bool(t.from_cache)}
 
 
 class Context:
@@ -178,6 +186,22 @@ class Context:
         """Put a snapshot of a context with the same setup back (gy_snapshot_load)."""
         buf = np.ascontiguousarray(buf, np.uint8)
         self._check(self.lib.gy_snapshot_load(self.h, buf.ctypes.data_as(_P), buf.size))
+
+    def autotune_shapes(self, trial_steps: int = 10, cache_path: str | None = None) -> dict:
+        """gy_autotune_shapes: tune (or take from `cache_path`) and apply the launch shapes."""
+        t = GyTuning()
+        self._check(self.lib.gy_autotune_shapes(self.h, trial_steps, cache_path.encode() if cache_path else None,
+                                                C.byref(t)))
+        return tuning_dict(t)
+
+    def tuning(self) -> dict:
+        t = GyTuning()
+        self._check(self.lib.gy_get_tuning(self.h, C.byref(t)))
+        return tuning_dict(t)
+
+    def set_tuning(self, persist_wgs: int, fused_waves: int = 3, grid_wgs_factor: int = 2) -> None:
+        t = GyTuning(persist_wgs=persist_wgs, fused_waves=fused_waves, grid_wgs_factor=grid_wgs_factor)
+        self._check(self.lib.gy_set_tuning(self.h, C.byref(t)))
 
     def audit(self) -> dict:
         a = GyAudit()
@@ -280,8 +304,19 @@ class BatchContext:
         self._check(self.lib.gy_download_inst(self.h, i, C.byref(pa.view())))
         return pa.arrays
 
+    def autotune_shapes(self, trial_steps: int = 10, cache_path: str | None = None) -> dict:
+        return autotune_shapes(self, trial_steps, cache_path)
+
     def autotune(self, trial_steps: int = 20):
         return autotune(self, trial_steps)
+
+
+def autotune_shapes(ctx, trial_steps: int = 10, cache_path: str | None = None) -> dict:
+    """gy_autotune_shapes on any context (single or batched)."""
+    t = GyTuning()
+    ctx._check(ctx.lib.gy_autotune_shapes(ctx.h, trial_steps, cache_path.encode() if cache_path else None,
+                                          C.byref(t)))
+    return tuning_dict(t)
 
 
 def autotune(ctx, trial_steps: int = 20) -> dict:

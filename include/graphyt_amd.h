@@ -251,10 +251,27 @@ int  gy_upload_batch(gy_ctx *ctx, int32_t ninst, const gy_particles *views);
 int  gy_download_inst(gy_ctx *ctx, int32_t inst, gy_particles *host);
 double gy_get_dt_inst(const gy_ctx *ctx, int32_t inst);
 
-/* Autotuner (no counterpart in the reference): times `trial_steps` steps for each
- * candidate workgroup count of the persistent particle kernel (1, 2, 3, 4 and 6 per
- * compute unit), keeps the fastest, and restores the particle state it started from.
- * Reports the choice and its time per step. */
+/* Autotuner (no counterpart in the reference, whose OpenMP schedule is fixed). Launch
+ * shapes of the step, per workload: */
+typedef struct gy_tuning {
+  int64_t persist_wgs;      /* workgroups of the persistent particle kernel               */
+  int32_t fused_waves;      /* build of the 3D single-body fused kernel: 3 resident waves
+                               per SIMD (168 VGPRs) or 2 (256 VGPRs)                       */
+  int32_t grid_wgs_factor;  /* grid-kernel workgroups per persistent workgroup: 1, 2 or 4  */
+  double  ms_per_step;      /* the trial time of this choice (tuning runs)                 */
+  int32_t from_cache;       /* 1: read from the cache file, no trials run                  */
+  int32_t reserved;
+} gy_tuning;
+/* Times `trial_steps` steps for each candidate (workgroups 1, 2, 3, 4, 6 per CU x both
+ * builds, then the grid factor), every trial from the same device state (a snapshot that
+ * is put back afterwards), and applies the fastest. With a cache file (may be NULL) a
+ * previous result for the same workload shape (device, instances, particles, nodes, tiles,
+ * model set, contact and heat modes) is applied without trials, and a new result is added
+ * to it (one text line per shape). */
+int  gy_autotune_shapes(gy_ctx *ctx, int32_t trial_steps, const char *cache_path, gy_tuning *out);
+int  gy_get_tuning(const gy_ctx *ctx, gy_tuning *out);
+int  gy_set_tuning(gy_ctx *ctx, const gy_tuning *t);
+/* The same tuner without a cache, reporting only the workgroup count and time per step. */
 int  gy_autotune(gy_ctx *ctx, int32_t trial_steps, int64_t *best_persist_wgs, double *best_ms_per_step);
 
 /* ---- learned Drucker-Prager return map -------------------------------------------

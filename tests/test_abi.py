@@ -44,6 +44,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(abi.GyParams) == 8 * 8 + 8
     assert C.sizeof(abi.GyParticles) == 8 + 21 * 8
     assert C.sizeof(abi.GyAudit) == 6 * 8 + 2 * 8
+    assert C.sizeof(abi.GyTuning) == 8 + 4 + 4 + 8 + 4 + 4
 
 
 def test_bc_codes_match_reference_enum():

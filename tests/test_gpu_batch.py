@@ -68,3 +68,45 @@ def test_autotune_restores_the_particle_state():
         assert rel(b.download(i, ["pos"])["pos"], ref.download(i, ["pos"])["pos"]) <= 1e-7
     b.close()
     ref.close()
+
+
+def test_autotune_shapes_cache_is_reused(tmp_path):
+    """gy_autotune_shapes tunes the workgroup count, the fused kernel's build and the grid
+    kernel's workgroups; the winner goes to the cache file under the workload's shape, and
+    a second context of the same shape takes it from there without trials. Tuning leaves
+    the state as it was (bit for bit) and the tuned launches give the default's results."""
+    cache = str(tmp_path / "tune.txt")
+    sc = W.dp_column_small(steps=60)
+    a = runtime.Context(sc)
+    a.run(10)
+    before = a.download(["pos", "vel", "stress"])
+    t1 = a.autotune_shapes(trial_steps=5, cache_path=cache)
+    print("tuned:", t1, flush=True)
+    assert not t1["from_cache"] and t1["persist_wgs"] % 8 == 0 and t1["ms_per_step"] > 0
+    assert t1["fused_waves"] in (2, 3) and t1["grid_wgs_factor"] in (1, 2, 4)
+    assert a.tuning()["persist_wgs"] == t1["persist_wgs"]
+    after = a.download(["pos", "vel", "stress"])
+    for k in before:
+        assert np.array_equal(before[k], after[k])
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("v1|")
+    a.run(40, step0=10)
+    tuned = a.download(["pos", "vel", "stress"])
+    a.close()
+    b = runtime.Context(sc)
+    t2 = b.autotune_shapes(trial_steps=5, cache_path=cache)
+    assert t2["from_cache"]
+    for k in ("persist_wgs", "fused_waves", "grid_wgs_factor"):
+        assert t2[k] == t1[k]
+    b.close()
+    ref = runtime.Context(sc)
+    ref.run(50)
+    r = ref.download(["pos", "vel", "stress"])
+    ref.close()
+    assert rel(tuned["pos"], r["pos"]) <= 1e-9
+    assert rel(tuned["vel"], r["vel"]) <= 1e-6 and rel(tuned["stress"], r["stress"]) <= 1e-5
+    # another shape (more particles) is not served from this entry
+    c = runtime.Context(W.dp_column_mid(steps=10))
+    assert not c.autotune_shapes(trial_steps=2, cache_path=cache)["from_cache"]
+    c.close()
+    assert len(open(cache).read().splitlines()) == 2
