@@ -43,7 +43,7 @@ class GyMaterial(C.Structure):
         ("psi_deg", C.c_double), ("crack_speed", C.c_double), ("damage_m", C.c_double),
         ("damage_k", C.c_double), ("thermal_conductivity", C.c_double),
         ("specific_heat", C.c_double), ("turb_stress", C.c_double),
-        ("dp_apex_guard", C.c_int32), ("reserved", C.c_int32),
+        ("dp_apex_guard", C.c_int32), ("dp_return", C.c_int32),
     ]
 
 
@@ -136,7 +136,7 @@ def material(model=1, bulk_mod=1e7, shear_mod=5000.0, density=1000.0, is_rigid=F
              viscosity=1.0, yield_stress=1e7, cohesion=1e5, critical_strain=0.01, phi=40.0,
              psi=10.0, is_damage=False, damage_model=1, crack_speed=586.52, damage_m=9.0,
              damage_k=5.39e36, thermal_conductivity=391.0, specific_heat=385.2,
-             turb_stress=1e10, dp_apex_guard=False) -> GyMaterial:
+             turb_stress=1e10, dp_apex_guard=False, dp_return=0) -> GyMaterial:
     """A gy_material with the reference defaults (MaterialModels.cpp:7-25)."""
     m = GyMaterial()
     m.model, m.is_rigid, m.is_damage, m.damage_model = model, int(is_rigid), int(is_damage), damage_model
@@ -146,6 +146,7 @@ def material(model=1, bulk_mod=1e7, shear_mod=5000.0, density=1000.0, is_rigid=F
     m.damage_m, m.damage_k = damage_m, damage_k
     m.thermal_conductivity, m.specific_heat, m.turb_stress = thermal_conductivity, specific_heat, turb_stress
     m.dp_apex_guard = int(dp_apex_guard)
+    m.dp_return = int(dp_return)  # 0 closed form, 1 the reference's loop, 2 learned
     return m
 
 

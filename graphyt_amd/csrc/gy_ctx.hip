@@ -73,6 +73,7 @@ struct gy_ctx {
   int64_t launches[K_COUNT_] = {0};
   double total_ms[K_COUNT_] = {0};
   std::vector<hipEvent_t> event_pool;
+  void *nnbuf = nullptr;  // network of the learned return map: fragments, biases, normalisation
 };
 
 namespace {
@@ -161,6 +162,8 @@ MatDev make_mat(const gy_material &m) {
   M.damage = m.is_damage != 0;
   M.gk = (m.is_damage != 0) && m.damage_model == 1;
   M.apex_guard = m.dp_apex_guard != 0;
+  M.dp_mode = m.model == 4 ? m.dp_return : GY_DP_RETURN_CLOSED;
+  M.phi_deg = (float)m.phi_deg;
   M.carry_plastic = (m.model == 3 || m.model == 4);
   M.carry_damage = M.gk || m.model == 5;  // model 5 keeps its turbulence flag in the damage field
   M.turb = (float)m.turb_stress;
@@ -221,6 +224,13 @@ int describe(gy_ctx *c, const gy_grid *g, const gy_material *mats, int nmat, con
   for (int i = 0; i < nmat; i++) {
     const int md = mats[i].model;
     if (md < 1 || md > 5) return fail(c, GY_ERR_UNSUPPORTED, "material %d: model %d not implemented", i, md);
+    const int dr = mats[i].dp_return;
+    if (dr < GY_DP_RETURN_CLOSED || dr > GY_DP_RETURN_LEARNED)
+      return fail(c, GY_ERR_INVALID, "material %d: dp_return %d is not a return-map mode", i, dr);
+    if (dr != GY_DP_RETURN_CLOSED && md != 4)
+      return fail(c, GY_ERR_INVALID, "material %d: dp_return applies to the Drucker-Prager model (4) only", i);
+    if (dr == GY_DP_RETURN_LEARNED && mats[i].is_damage)
+      return fail(c, GY_ERR_UNSUPPORTED, "material %d: the learned return map is built for is_damage = 0", i);
   }
   H.grid = *g;
   H.params = *p;
@@ -663,6 +673,9 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
       kmask |= gm.model == 1 ? 1 : gm.model == 2 ? 2 : gm.model == 3 ? 4 : gm.model == 4 ? 8 : 64;
       if (gm.is_damage && gm.damage_model == 1) kmask |= 16;
       if (c->mats[I.mat_base + m].carry_damage && !(gm.is_damage && gm.damage_model == 1)) kmask |= 16;
+      // other Drucker-Prager multipliers: the reference's loop 128, the network 256
+      if (gm.model == 4 && gm.dp_return == GY_DP_RETURN_REFERENCE) kmask |= 128;
+      if (gm.model == 4 && gm.dp_return == GY_DP_RETURN_LEARNED) kmask |= 256;
     }
     for (int i = 0; in->bc_type && i < n; i++)
       if (in->bc_type[i]) { kmask |= 32; break; }
@@ -671,6 +684,9 @@ static int upload_impl(gy_ctx *c, int32_t ninst, const gy_particles *views) {
     I.pcount = n;
     pstart += n;
   }
+  if ((kmask & (128 | 256)) && (c->plane || multibody))
+    return fail(c, GY_ERR_UNSUPPORTED, "dp_return %s runs in 3D single-body contexts only",
+                (kmask & 256) ? "GY_DP_RETURN_LEARNED" : "GY_DP_RETURN_REFERENCE");
   d.kmask = kmask;
   d.multibody = multibody ? 1 : 0;
   d.thermal = thermal ? 1 : 0;
@@ -727,6 +743,8 @@ static int iterate_impl(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
   if (!c) return GY_ERR_INVALID;
   if (!c->uploaded) return fail(c, GY_ERR_STATE, "no particles uploaded");
   if (nsteps <= 0) return GY_OK;
+  if ((c->d.kmask & 256) && !c->d.nn.frag)
+    return fail(c, GY_ERR_STATE, "a material uses the learned return map: gy_set_return_network first");
   launch_set_ctrl(c->d, step0, t0, 0, c->stream);
   if (!c->acc_valid) {
     enqueue_init(c);
@@ -1462,6 +1480,43 @@ int gy_reset_kernel_stats(gy_ctx *c) {
 
 void *gy_stream(gy_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int gy_set_return_network(gy_ctx *c, gy_mlp *m) {
+  if (!c || !m) return GY_ERR_INVALID;
+  std::vector<uint2> frag(NN_TILES * 64);
+  std::vector<float> bias(NN_NBIAS), norm(NN_NORM_N);
+  int rc = mlp_export(m, frag.data(), bias.data(), norm.data());
+  if (rc) return fail(c, rc, "network export failed: %s", gy_mlp_last_error(m));
+  const size_t fb = frag.size() * sizeof(uint2), bb = bias.size() * sizeof(float), nb = norm.size() * sizeof(float);
+  if (!c->nnbuf) {
+    HIPCHECK(c, hipMalloc(&c->nnbuf, fb + bb + nb));
+    char *base = (char *)c->nnbuf;
+    c->d.nn.frag = (const uint2 *)base;
+    c->d.nn.bias = (const float *)(base + fb);
+    c->d.nn.norm = (const float *)(base + fb + bb);
+    drop_graph(c);  // the captured launches hold the old pointers
+  }
+  HIPCHECK(c, hipMemcpyAsync((void *)c->d.nn.frag, frag.data(), fb, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync((void *)c->d.nn.bias, bias.data(), bb, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync((void *)c->d.nn.norm, norm.data(), nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return GY_OK;
+}
+
+int gy_return_map_counts(gy_ctx *c, int64_t *network, int64_t *fallback, int32_t reset) {
+  if (!c) return GY_ERR_INVALID;
+  if (!c->d.ctrl) return fail(c, GY_ERR_STATE, "no particles uploaded");
+  unsigned long long n[2] = {0, 0};
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(n, c->d.ctrl->nn_count, sizeof n, hipMemcpyDeviceToHost));
+  if (network) *network = (int64_t)n[0];
+  if (fallback) *fallback = (int64_t)n[1];
+  if (reset) {
+    const unsigned long long z[2] = {0, 0};
+    HIPCHECK(c, hipMemcpy(c->d.ctrl->nn_count, z, sizeof z, hipMemcpyHostToDevice));
+  }
+  return GY_OK;
+}
+
 void gy_destroy(gy_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1476,6 +1531,7 @@ void gy_destroy(gy_ctx *c) {
   dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.nodemask); dfree(d.pcore); dfree(d.partial);
   dfree(d.acc2); dfree(d.nrm); dfree(d.claim); dfree(d.velb); dfree(d.cellobj); dfree(d.twotile);
   dfree(d.therm);
+  dfree(c->nnbuf);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

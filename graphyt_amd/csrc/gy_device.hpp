@@ -149,6 +149,10 @@ __device__ __forceinline__ void node_terms(const P2GTerms &T, float w, float gx,
 constexpr int KM_ELASTIC = 1, KM_NEWTON = 2, KM_PERFECT = 4, KM_DP = 8, KM_GK = 16, KM_PBC = 32,
               KM_TURB = 64;
 constexpr int KM_ALL = 127;
+// Drucker-Prager multipliers other than the closed form, outside KM_ALL: the reference's
+// loop control (a material with dp_return = GY_DP_RETURN_REFERENCE) and the network
+// (GY_DP_RETURN_LEARNED); single-body 3D kernels only
+constexpr int KM_DPLOOP = 128, KM_DPNN = 256;
 
 // Rank of a lane's key among the equal keys of its wave, with one global atomic per run
 // of equal keys (lanes hold consecutive entries of a cell-grouped list, so equal keys

@@ -24,6 +24,9 @@
 #pragma once
 #include <cstdint>
 
+#include "../../include/graphyt_amd.h"
+#include "gy_nn.hpp"
+
 struct gy_material;
 
 namespace gy {
@@ -119,6 +122,8 @@ struct MatDev {
   int apex_guard;
   int carry_plastic;  // plastic strain fields are live for this material
   int carry_damage;   // damage / smax fields are live for this material (Grady-Kipp, model 5's flag)
+  int dp_mode;        // Drucker-Prager multipliers: GY_DP_RETURN_CLOSED / _REFERENCE / _LEARNED
+  float phi_deg;      // friction angle (degrees), the network's material input
   float K, G, visc, sy;
   float turb;         // turbStress of model 5
   float cp, kt;       // specificHeatCapacity, thermalConductivity
@@ -144,6 +149,9 @@ struct Ctrl {
   // workgroup adds to it, and loads of fields sharing its line stall behind those atomics
   alignas(256) int next;
   int active_bricks;  // length of the flagged-brick list (appended with atomics)
+  // learned return map: yielding samples through the network, and outside its trained
+  // range (closed form), since the upload or the last reset
+  unsigned long long nn_count[2];
 };
 
 // All device pointers of a context (passed by value to kernels).
@@ -202,6 +210,7 @@ struct DevPtrs {
   int thermal;
   float *therm;          // 2 per node: heat content sum(N m cp T), conduction rate
   long long nnodes;      // nodes of all instances
+  NetDev nn;             // network of the learned return map (null pointers when none)
 };
 
 constexpr int MB_ACC2_W = 16;   // per canonical node: [m p3 f3 mr] for body 1, then body 2
@@ -215,6 +224,9 @@ enum KernelId : int {
 
 // Device material constants of a gy_material (defined in gy_ctx.hip).
 MatDev material_constants(const gy_material &m);
+// A network's parameters in the step's layout (gy_nn.hpp): NN_TILES * 64 fragments,
+// NN_NBIAS biases, NN_NORM_N normalisation values (defined in gy_mlp.hip).
+int mlp_export(gy_mlp *m, uint2 *frag, float *bias, float *norm);
 
 // Host-side launchers (defined in gy_kernels.hip). Every kernel reads the step number and
 // the buffer parity from Ctrl, so a captured sequence replays correctly step after step.

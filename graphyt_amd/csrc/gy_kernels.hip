@@ -42,7 +42,10 @@ __global__ void k_set_ctrl(DevPtrs d, int step, double t, int reset) {
   if (i < d.ninst) d.t_cur[i] = t;
   if (i == 0) {
     d.ctrl->step = step;
-    if (reset) { d.ctrl->parity = 0; d.ctrl->status = 0; d.ctrl->oor = 0; }
+    if (reset) {
+      d.ctrl->parity = 0; d.ctrl->status = 0; d.ctrl->oor = 0;
+      d.ctrl->nn_count[0] = 0; d.ctrl->nn_count[1] = 0;
+    }
     if (d.ctrl->epoch == 0) d.ctrl->epoch = 1;  // tile_epoch starts at 0: never current
   }
 }
@@ -489,6 +492,52 @@ struct RecLds {
 
 
 
+// A yielding Drucker-Prager sample of a learned material, held between phase A and the
+// wavefront's network evaluation (k_g2p2g with KM_DPNN).
+struct DpPend {
+  int state;        // 0 none, 1 waiting for the network, 2 outside the trained range (closed form taken)
+  int mat;          // material index
+  int slot;         // source slot (the stress and plastic fields before the step are read there again)
+  uint32_t x[8];    // normalised network inputs, BF16 pairs
+  DpTrial T;        // elastic trial state
+  float nV;         // -m/rho of the record
+};
+
+// Learned return map, phase A part: the trial state exactly; an elastic sample takes the
+// trial stress; a yielding one inside the trained input range waits for the network
+// (s becomes the trial stress until then); outside it, the closed form.
+__device__ __forceinline__ void dp_learned_gate(const float *norm, const MatDev &M, int mat, int slot,
+                                                const float er[6], float s[6], float p[6], float &epb, float dt,
+                                                DpPend &pd) {
+  DpTrial T;
+  dp_trial(M, er, s, dt, T);
+  if (!T.yield) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = T.tr[k];
+    return;
+  }
+  float xr[GY_MLP_IN];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    xr[k] = er[k] * dt;
+    xr[6 + k] = s[k];
+  }
+  xr[12] = epb;
+  xr[13] = M.phi_deg;
+  if (nn_inputs(norm, xr, pd.x)) {
+    pd.state = 1;
+    pd.mat = mat;
+    pd.slot = slot;
+    pd.T = T;
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = T.tr[k];
+    return;
+  }
+  pd.state = 2;
+  const float dL = dp_cone_multiplier<false>(M, T, epb);
+  dp_return<false>(M, T, dL, s, p, epb);
+}
+
 // Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
 // constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
 // state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
@@ -504,7 +553,8 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
                                                const MatDev *s_mat,
                                                int par, RecLds *s_rec, int j,
                                                const float4 (*s_vel2)[2] = nullptr,
-                                               int (*s_obj)[NODES_G2P] = nullptr) {
+                                               int (*s_obj)[NODES_G2P] = nullptr, DpPend *pd = nullptr) {
+  constexpr bool DPNN = (MASK & KM_DPNN) != 0, DPLOOP = (MASK & KM_DPLOOP) != 0;
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
   const long long cap = d.cap;
@@ -641,10 +691,17 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         for (int a = 0; a < 6; a++) p[a] = f32(ld_stream(&A[(F_P0 + a) * cap + slot]));
         float epb = f32(ld_stream(&A[F_EPB * cap + slot]));
         if ((MASK & KM_PERFECT) && M.model == 3) cm_perfect(M, er, s, p, epb, dt);
-        if ((MASK & KM_DP) && M.model == 4) cm_drucker_prager(M, er, s, p, epb, dt);
+        if ((MASK & KM_DP) && M.model == 4) {
+          if (DPNN && M.dp_mode == GY_DP_RETURN_LEARNED)
+            dp_learned_gate(d.nn.norm, M, tag_mat(tag), slot, er, s, p, epb, dt, *pd);
+          else
+            cm_drucker_prager<DPLOOP>(M, er, s, p, epb, dt);
+        }
+        if (!DPNN || pd->state != 1) {  // a sample waiting for the network stores after it
 #pragma unroll
-        for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = u32(p[a]);
-        B[F_EPB * cap + out] = u32(epb);
+          for (int a = 0; a < 6; a++) B[(F_P0 + a) * cap + out] = u32(p[a]);
+          B[F_EPB * cap + out] = u32(epb);
+        }
       }
       if (MASK & (KM_GK | KM_TURB)) {
         if (M.carry_damage) {
@@ -700,8 +757,10 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     for (int a = 0; a < 3; a++) B[(F_VX + a) * cap + out] = u32(v[a]);
 #pragma unroll
     for (int a = 0; a < 3; a++) B[(F_XV0 + a) * cap + out] = xv[a];
+    if (!DPNN || pd->state != 1) {
 #pragma unroll
-    for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
+      for (int a = 0; a < 6; a++) B[(F_S0 + a) * cap + out] = u32(s[a]);
+    }
     B[F_RHO * cap + out] = u32(rho);
     B[F_MASS * cap + out] = u32(mass);
     B[F_TAG * cap + out] = tag;
@@ -724,6 +783,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
   // ---- P2G record of the (new) state (Extrapolation.cpp:166-286) ----
   const float mp = M.rigid ? 1.0f : mass;  // rigid particles scatter unit mass (Extrapolation.cpp:183)
   const float nV = -(mp / rho);
+  if (DPNN) pd->nV = nV;
   float4 *w = &s_rec->w[RecLds::wi(j)];
   w[0] = make_float4(__int_as_float(q[0]), __int_as_float(q[1]), __int_as_float(q[2]), mp);
   w[1] = make_float4(mp * v[0], mp * v[1], mp * v[2], nV * s[0]);
@@ -819,6 +879,44 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
   else p2g_global(d.acc, d.brick_flag, &d.inst[inst], P.b[0], P.b[1], P.b[2], P.W[0], P.W[1], P.W[2], P.T);
 }
 
+// Learned return map, after the network: the cone multiplier from the network output y
+// (the closed form near the return's branch switch), the return along the exact
+// direction (apex return exact), the stores of the stress and plastic fields at sorted
+// position out, and the stress terms of the record in column j. Returns whether the
+// network's multiplier was used.
+__device__ __forceinline__ bool learned_return(const DevPtrs &d, const MatDev *s_mat, const DpPend &pd, float y,
+                                               int par, int out, RecLds *s_rec, int j) {
+  const MatDev &M = s_mat[pd.mat];
+  const float *norm = d.nn.norm;
+  float dL = dp_learned_multiplier(M, pd.T, norm[NN_NORM_OUT] * y + norm[NN_NORM_OUT + 1], norm[NN_NORM_YLO],
+                                   norm[NN_NORM_YHI]);
+  const bool used = dL >= 0.0f;
+  // the stress and plastic fields before the step, from the source buffer
+  const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
+  uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
+  const long long cap = d.cap;
+  float s[6], p[6], epb = f32(ld_stream(&A[F_EPB * cap + pd.slot]));
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    s[k] = f32(ld_stream(&A[(F_S0 + k) * cap + pd.slot]));
+    p[k] = f32(ld_stream(&A[(F_P0 + k) * cap + pd.slot]));
+  }
+  if (!used) dL = dp_cone_multiplier<false>(M, pd.T, epb);
+  dp_return<false>(M, pd.T, dL, s, p, epb);
+#pragma unroll
+  for (int a = 0; a < 6; a++) {
+    B[(F_S0 + a) * cap + out] = u32(s[a]);
+    B[(F_P0 + a) * cap + out] = u32(p[a]);
+  }
+  B[F_EPB * cap + out] = u32(epb);
+  const float nV = pd.nV;
+  float4 *w = &s_rec->w[RecLds::wi(j)];
+  w[1].w = nV * s[0];
+  w[2] = make_float4(nV * s[1], nV * s[2], nV * s[3], nV * s[4]);
+  s_rec->t[RecLds::ti(j)] = nV * s[5];
+  return used;
+}
+
 // The fused particle kernel, persistent over the active tile list. UPDATE = false: P2G of
 // the current state only (first step after an upload). UPDATE = true: G2P + update +
 // constitutive + P2G of the new state.
@@ -853,6 +951,8 @@ template <bool UPDATE, int MASK, bool MB = false, int WAVES = 3>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAVES))) void k_g2p2g(DevPtrs d) {
   static_assert(WG == 4 * TILE3, "four waves per tile");
   static_assert(!MB || UPDATE, "the contact variant moves particles");
+  constexpr bool DPNN = (MASK & KM_DPNN) != 0;
+  static_assert(!DPNN || (UPDATE && !MB), "the network runs in the single-body stepping kernel");
   constexpr int NB2 = MB ? NODES_G2P : 1;
   __shared__ float4 s_vel[NODES_G2P][2];
   __shared__ float4 s_vel2[NB2][2];
@@ -864,10 +964,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
   __shared__ int s_tile;
   __shared__ int s_refkey[WG / 64];
   __shared__ float4 s_refv[WG / 64];
+  __shared__ uint2 s_nnw[DPNN ? NN_TILES * 64 : 1];               // network fragments
+  __shared__ __align__(16) float s_nnb[DPNN ? NN_NBIAS : 4];       // network biases
   const int tid = threadIdx.x;
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
   const TileQueue queue(d);
+  unsigned nn_net = 0, nn_fb = 0;  // this wave's network and fallback samples
+  if constexpr (DPNN) nn_stage(d.nn, s_nnw, s_nnb);  // visible after the loop's first barrier
 
   for (;;) {
     if (tid == 0) {
@@ -877,7 +981,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
     // workgroup-uniform values read from LDS: keep them (and all that derives from
     // them, the instance description included) in scalar registers
     const int t = __builtin_amdgcn_readfirstlane(s_tile);
-    if (t < 0) break;
+    if (t < 0) {
+      if (DPNN && (tid & 63) == 0 && (nn_net | nn_fb)) {
+        atomicAdd(&d.ctrl->nn_count[0], (unsigned long long)nn_net);
+        atomicAdd(&d.ctrl->nn_count[1], (unsigned long long)nn_fb);
+      }
+      break;
+    }
     const int k0 = t * TILE3;
     const int ii = d.tile_inst[t];
     const InstDesc I = d.inst[ii];
@@ -978,10 +1088,32 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       const int col = 16 * (j - jq * P) + jq;
       // phase A
       int knew = -1;
+      DpPend pd;
+      if constexpr (DPNN) {
+        pd.state = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) pd.x[k] = 0u;
+      }
       if (i < c1)
         knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, vref, s_mat, par, &s_rec, col,
                                                  MB ? s_vel2 : nullptr,
-                                                 MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr);
+                                                 MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr,
+                                                 DPNN ? &pd : nullptr);
+      if constexpr (DPNN) {
+        // the wavefront's waiting samples through the network (all lanes take part), then
+        // their return, stores and record
+        const unsigned long long wait = __ballot(pd.state == 1);
+        bool closed = pd.state == 2;
+        if (wait) {
+          // all four 16-sample blocks interleaved: more independent matrix-core and SiLU
+          // work in flight (1.21 against 1.27 ms per step on the 4.2M-particle column)
+          const float y = nn_wave_forward<4>(s_nnw, s_nnb, pd.x, wait);
+          if (pd.state == 1) closed = !learned_return(d, s_mat, pd, y, par, i, &s_rec, col);
+        }
+        const unsigned nfb = (unsigned)__popcll(__ballot(closed));
+        nn_fb += nfb;
+        nn_net += (unsigned)__popcll(wait) + (unsigned)__popcll(__ballot(pd.state == 2)) - nfb;
+      }
       int hl = 0, hbase = 0;
       if (UPDATE) hbase = hist_issue(d.count, knew, hl);
       __syncthreads();
@@ -1471,8 +1603,19 @@ void launch_place(const DevPtrs &d, void *st) {
 // Smallest built model-set specialization covering the context's mask.
 template <bool U, int W>
 void launch_fused_w(const DevPtrs &d, hipStream_t st) {
-  const int m = d.kmask;
+  const int m = U ? d.kmask : (d.kmask & KM_ALL);  // the P2G of an upload runs no constitutive model
   const dim3 g(d.persist_wgs), b(WG);
+  if constexpr (U) {
+    if (m & (KM_DPLOOP | KM_DPNN)) {
+      // other Drucker-Prager multipliers; the network's LDS leaves two workgroups per CU
+      // (the default three per CU scaled to two)
+      const dim3 gn(std::max(8, (d.persist_wgs * 2 / 3 + 7) / 8 * 8));
+      if (m == (KM_DP | KM_DPLOOP)) hipLaunchKernelGGL((k_g2p2g<true, KM_DP | KM_DPLOOP, false, W>), g, b, 0, st, d);
+      else if (m == (KM_DP | KM_DPNN)) hipLaunchKernelGGL((k_g2p2g<true, KM_DP | KM_DPNN, false, 2>), gn, b, 0, st, d);
+      else hipLaunchKernelGGL((k_g2p2g<true, KM_ALL | KM_DPLOOP | KM_DPNN, false, 2>), gn, b, 0, st, d);
+      return;
+    }
+  }
   if ((m & ~KM_DP) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_DP, false, W>), g, b, 0, st, d);
   else if ((m & ~KM_ELASTIC) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_ELASTIC, false, W>), g, b, 0, st, d);
   else if ((m & ~KM_PERFECT) == 0) hipLaunchKernelGGL((k_g2p2g<U, KM_PERFECT, false, W>), g, b, 0, st, d);

@@ -123,88 +123,194 @@ __device__ __forceinline__ void cm_perfect(const MatDev &M, const float er[6], f
   s[5] += (2.0f * G * el[5]) * dt;
 }
 
-// druckerPrager_Damage, ConstitutiveModels.cpp:238-428. The two Newton loops solve
-// piecewise-linear equations; the reference iterates to |F/c| <= 1e-8 or 201 passes, this
-// FP32 form iterates to its fixed point (at most 16 passes) — the same root to rounding.
-__device__ __forceinline__ void cm_drucker_prager(const MatDev &M, const float er[6], float s[6],
-                                                  float p[6], float &epb, float dt) {
+// druckerPrager_Damage, ConstitutiveModels.cpp:238-428, in three parts: the elastic
+// trial state and yield test (dp_trial), the multipliers of the cone and apex returns
+// (Newton on piecewise-linear consistency conditions), and the return from the trial
+// state given the cone multiplier (dp_return). Multipliers, per material
+// (gy_material::dp_return):
+//   GY_DP_RETURN_CLOSED     Newton to the iterate's fixed point (at most 16 passes): the
+//                           reference's root to rounding;
+//   GY_DP_RETURN_REFERENCE  the reference's loop control, while |Fp/cval| > 1e-8 and at
+//                           most 201 passes (with c = 0 the test divides by cval = 0 and
+//                           every pass runs);
+//   GY_DP_RETURN_LEARNED    the cone multiplier from the network (gy_kernels.hip), the
+//                           apex multiplier as GY_DP_RETURN_CLOSED.
+struct DpTrial {
+  float tr[6];     // elastic trial stress (ConstitutiveModels.cpp:270-276)
+  float I1, J2, sJ2, F;
+  bool yield;
+};
+
+__device__ __forceinline__ void dp_trial(const MatDev &M, const float er[6], const float s[6], float dt,
+                                         DpTrial &T) {
   const float G = M.G, K = M.K;
   const float vol = 0.3333f * (er[0] + er[1] + er[2]);
-  float tr[6];
-  tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
-  tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
-  tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
-  tr[3] = s[3] + (2.0f * G * er[3]) * dt;
-  tr[4] = s[4] + (2.0f * G * er[4]) * dt;
-  tr[5] = s[5] + (2.0f * G * er[5]) * dt;
-  const float I1 = (1.0f / 3.0f) * (tr[0] + tr[1] + tr[2]);
-  const float sxx = tr[0] - I1, syy = tr[1] - I1, szz = tr[2] - I1;
-  const float sxy = tr[3], syz = tr[4], sxz = tr[5];
-  const float J2 = 0.5f * (sxx * sxx + syy * syy + szz * szz + 2.0f * sxy * sxy + 2.0f * syz * syz +
-                           2.0f * sxz * sxz);
-  const float sJ2 = sqrtf(J2);
-  const float F = sJ2 + M.eta * I1 - M.xi * M.c;
-  const bool yield = M.damage ? ((F > 0.0f) && (I1 < 0.0f)) : (F > 0.0f);
-  if (!yield) {
-    for (int k = 0; k < 6; k++) s[k] = tr[k];
-    return;
-  }
-  const float ep0 = epb;
-  float dL = 0.0f;
+  T.tr[0] = s[0] + (2.0f * G * (er[0] - vol) + 3.0f * K * vol) * dt;
+  T.tr[1] = s[1] + (2.0f * G * (er[1] - vol) + 3.0f * K * vol) * dt;
+  T.tr[2] = s[2] + (2.0f * G * (er[2] - vol) + 3.0f * K * vol) * dt;
+  T.tr[3] = s[3] + (2.0f * G * er[3]) * dt;
+  T.tr[4] = s[4] + (2.0f * G * er[4]) * dt;
+  T.tr[5] = s[5] + (2.0f * G * er[5]) * dt;
+  T.I1 = (1.0f / 3.0f) * (T.tr[0] + T.tr[1] + T.tr[2]);
+  const float sxx = T.tr[0] - T.I1, syy = T.tr[1] - T.I1, szz = T.tr[2] - T.I1;
+  T.J2 = 0.5f * (sxx * sxx + syy * syy + szz * szz + 2.0f * T.tr[3] * T.tr[3] + 2.0f * T.tr[4] * T.tr[4] +
+                 2.0f * T.tr[5] * T.tr[5]);
+  T.sJ2 = sqrtf(T.J2);
+  T.F = T.sJ2 + M.eta * T.I1 - M.xi * M.c;
+  T.yield = M.damage ? ((T.F > 0.0f) && (T.I1 < 0.0f)) : (T.F > 0.0f);
+}
+
+// Cone multiplier dL (ConstitutiveModels.cpp:313-343); ep0: accumulated plastic strain
+// before the step.
+template <bool REF_LOOP>
+__device__ __forceinline__ float dp_cone_multiplier(const MatDev &M, const DpTrial &T, float ep0) {
+  const float G = M.G, K = M.K;
   const float slope0 = -G - K * M.eta * M.eta_bar;
+  float dL = 0.0f;
+  if (REF_LOOP) {
+    int it = 0;
+    float err = 1.0f;
+    while (err > 1e-8f) {
+      it++;
+      float cv = M.c - (ep0 + M.xi * dL) * M.c_over_ecr;
+      float dcv = -M.c_over_ecr;
+      if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
+      const float Fp = T.sJ2 - G * dL + M.eta * (T.I1 - M.eta_bar * K * dL) - M.xi * cv;
+      const float dFp = slope0 - M.xi * M.xi * dcv;
+      dL = dL - Fp / dFp;
+      err = fabsf(Fp / cv);
+      if (it > 200) break;
+    }
+    return dL;
+  }
   for (int it = 0; it < 16; it++) {
     float cv = M.c - (ep0 + M.xi * dL) * M.c_over_ecr;
     float dcv = -M.c_over_ecr;
     if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
-    const float Fp = sJ2 - G * dL + M.eta * (I1 - M.eta_bar * K * dL) - M.xi * cv;
+    const float Fp = T.sJ2 - G * dL + M.eta * (T.I1 - M.eta_bar * K * dL) - M.xi * cv;
     const float dFp = slope0 - M.xi * M.xi * dcv;
     const float dLn = dL - Fp / dFp;
     const bool done = (dLn == dL) || !(dLn == dLn);
     dL = dLn;
     if (done) break;
   }
-  if ((sJ2 - G * dL < 0.0f) && !M.damage) {
-    // apex return (ConstitutiveModels.cpp:347-395)
-    float de = 0.0f;
-    for (int it = 0; it < 16; it++) {
-      float cv = M.c - (ep0 + M.xi_over_eta * de) * M.c_over_ecr;
-      float dcv = -M.c_over_ecr;
-      if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
-      float Gc = cv * M.xi_over_etab, dGc = M.xi_over_eta * dcv * M.xi_over_etab;
-      if (M.apex_guard) {
-        if (cv == 0.0f) Gc = 0.0f;
-        if (dcv == 0.0f) dGc = 0.0f;
-      }
-      const float Gp = Gc - I1 + K * de;
-      const float den = de - Gp / (dGc + K);
+  return dL;
+}
+
+// Apex multiplier deps (ConstitutiveModels.cpp:347-375).
+template <bool REF_LOOP>
+__device__ __forceinline__ float dp_apex_multiplier(const MatDev &M, const DpTrial &T, float ep0) {
+  const float K = M.K;
+  float de = 0.0f;
+  int it = 0;
+  float err = 1.0f;
+  for (;;) {
+    if (REF_LOOP && !(err > 1e-8f)) break;
+    if (!REF_LOOP && it == 16) break;
+    it++;
+    float cv = M.c - (ep0 + M.xi_over_eta * de) * M.c_over_ecr;
+    float dcv = -M.c_over_ecr;
+    if (cv < M.c_floor) { cv = M.c_floor; dcv = 0.0f; }
+    float Gc = cv * M.xi_over_etab, dGc = M.xi_over_eta * dcv * M.xi_over_etab;
+    if (M.apex_guard) {
+      if (cv == 0.0f) Gc = 0.0f;
+      if (dcv == 0.0f) dGc = 0.0f;
+    }
+    const float Gp = Gc - T.I1 + K * de;
+    const float den = de - Gp / (dGc + K);
+    if (REF_LOOP) {
+      de = den;
+      err = fabsf(Gp / cv);
+      if (it > 200) break;
+    } else {
       const bool done = (den == de) || !(den == den);
       de = den;
       if (done) break;
     }
+  }
+  return de;
+}
+
+// The return from a yielding trial state with cone multiplier dL
+// (ConstitutiveModels.cpp:345-424): the apex return when sqrt(J2) - G dL < 0 (not for a
+// damage material), the cone return otherwise. s holds the stress before the step and
+// receives the new one; p and epb are updated. sqrt(J2) = 0 takes the apex return: a
+// yielding state has F > 0 and so dL > 0, which makes the test negative, but a trial
+// stress of ~1e-31 Pa (the velocity gradient of a body at rest, to rounding) leaves J2
+// and F / G below the FP32 normal range, flushed to zero, and the cone return would
+// divide 0 by sqrt(J2) = 0.
+template <bool REF_LOOP>
+__device__ __forceinline__ void dp_return(const MatDev &M, const DpTrial &T, float dL, float s[6], float p[6],
+                                          float &epb) {
+  const float G = M.G, K = M.K;
+  if ((T.sJ2 - G * dL < 0.0f || T.sJ2 == 0.0f) && !M.damage) {
+    const float de = dp_apex_multiplier<REF_LOOP>(M, T, epb);
     if (de < 0.0f) return;  // deps < 0: stress left as it was (ConstitutiveModels.cpp:376-378)
-    const float sm = I1 - K * de;
+    const float sm = T.I1 - K * de;
     s[0] = sm; s[1] = sm; s[2] = sm; s[3] = 0.0f; s[4] = 0.0f; s[5] = 0.0f;
     epb += de * M.xi_over_eta;
     const float pv = de * (M.eta_bar / 3.0f);
     p[0] += pv; p[1] += pv; p[2] += pv;
     return;
   }
-  const float r = 1.0f - G * dL / sJ2;
-  const float sm = I1 - K * M.eta_bar * dL;
+  const float sxx = T.tr[0] - T.I1, syy = T.tr[1] - T.I1, szz = T.tr[2] - T.I1;
+  const float r = 1.0f - G * dL / T.sJ2;
+  const float sm = T.I1 - K * M.eta_bar * dL;
   s[0] = r * sxx + sm;
   s[1] = r * syy + sm;
   s[2] = r * szz + sm;
-  s[3] = r * sxy;
-  s[4] = r * syz;
-  s[5] = r * sxz;
+  s[3] = r * T.tr[3];
+  s[4] = r * T.tr[4];
+  s[5] = r * T.tr[5];
   epb += dL * M.xi;
-  const float hj = 0.5f / J2, pv = M.eta_bar / 3.0f;
+  const float hj = 0.5f / T.J2, pv = M.eta_bar / 3.0f;
   p[0] += dL * (hj * sxx + pv);
   p[1] += dL * (hj * syy + pv);
   p[2] += dL * (hj * szz + pv);
-  p[3] += dL * hj * sxy;
-  p[4] += dL * hj * syz;
-  p[5] += dL * hj * sxz;
+  p[3] += dL * hj * T.tr[3];
+  p[4] += dL * hj * T.tr[4];
+  p[5] += dL * hj * T.tr[5];
+}
+
+// The cone multiplier of a learned material from the network's scaled output yr (the
+// multiplier over its linear estimate k = F / (G + K eta eta_bar)); negative when the gate
+// sends the sample to the closed form:
+//  * yr outside the trained output range [ylo, yhi] (an input combination the network
+//    extrapolates on);
+//  * the branch test sqrt(J2) - G dL < 0 of the return (ConstitutiveModels.cpp:346)
+//    giving different answers at the two ends of that range: near the switch an error of
+//    the network's size picks the other branch (at near-zero pressure a multiplier too
+//    large by 1e-5 sends a cone return to the apex, which keeps the old stress).
+__device__ __forceinline__ float dp_learned_multiplier(const MatDev &M, const DpTrial &T, float yr, float ylo,
+                                                       float yhi) {
+  if (!(yr >= ylo && yr <= yhi)) return -1.0f;
+  const float k = T.F / (M.G + M.K * M.eta * M.eta_bar);
+  const bool apex_lo = T.sJ2 - M.G * fmaxf(ylo * k, 0.0f) < 0.0f;
+  const bool apex_hi = T.sJ2 - M.G * fmaxf(yhi * k, 0.0f) < 0.0f;
+  if (apex_lo != apex_hi) return -1.0f;
+  return fmaxf(yr * k, 0.0f);
+}
+
+// The whole map with the closed or (LOOP and the material asks for it) the reference's
+// multipliers. A learned material reaching this function (outside the kernels that
+// evaluate the network) takes the closed form.
+template <bool LOOP = false>
+__device__ __forceinline__ void cm_drucker_prager(const MatDev &M, const float er[6], float s[6], float p[6],
+                                                  float &epb, float dt) {
+  DpTrial T;
+  dp_trial(M, er, s, dt, T);
+  if (!T.yield) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = T.tr[k];
+    return;
+  }
+  if (LOOP && M.dp_mode == GY_DP_RETURN_REFERENCE) {
+    const float dL = dp_cone_multiplier<true>(M, T, epb);
+    dp_return<true>(M, T, dL, s, p, epb);
+  } else {
+    const float dL = dp_cone_multiplier<false>(M, T, epb);
+    dp_return<false>(M, T, dL, s, p, epb);
+  }
 }
 
 // Symmetric 3x3 eigen-decomposition in FP64 by cyclic Jacobi rotations (Golub & Van Loan,

@@ -56,12 +56,14 @@ SIGNATURES = {
     "gy_mlp_param_count": (C.c_int, []),
     "gy_mlp_set_params": (C.c_int, [_P, _FP]),
     "gy_mlp_get_params": (C.c_int, [_P, _FP]),
-    "gy_mlp_set_norm": (C.c_int, [_P, _FP, _FP, _FP, _FP, _FP]),
+    "gy_mlp_set_norm": (C.c_int, [_P, _FP, _FP, _FP, _FP, C.c_float, C.c_float, C.c_float, C.c_float]),
     "gy_mlp_forward": (C.c_int, [_P, _FP, C.c_int64, _FP]),
     "gy_mlp_gradient": (C.c_int, [_P, _FP, _FP, C.c_int64, _FP, _DP]),
     "gy_mlp_load_data": (C.c_int, [_P, _FP, _FP, C.c_int64]),
     "gy_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_double, _DP]),
-    "gy_dp_map": (C.c_int, [C.POINTER(GyMaterial), _P, C.c_int32, _FP, _FP, C.c_int64, _I64P, _I64P, _DP]),
+    "gy_dp_map": (C.c_int, [C.POINTER(GyMaterial), _P, C.c_int32, _FP, _FP, _FP, C.c_int64, _I64P, _I64P, _DP]),
+    "gy_set_return_network": (C.c_int, [_P, _P]),
+    "gy_return_map_counts": (C.c_int, [_P, _I64P, _I64P, C.c_int32]),
     "gy_mlp_last_error": (C.c_char_p, [_P]),
     "gy_mlp_destroy": (None, [_P]),
     "gy_set_option": (C.c_int, [_P, C.c_int32, C.c_int64]),
@@ -238,6 +240,16 @@ class Context:
     def stream(self) -> int:
         return self.lib.gy_stream(self.h)
 
+    def set_return_network(self, net: "Mlp") -> None:
+        """gy_set_return_network: the network of materials with dp_return = learned."""
+        self._check(self.lib.gy_set_return_network(self.h, net.h))
+
+    def return_map_counts(self, reset: bool = False) -> dict:
+        """Learned-material samples through the network and through the closed-form fallback."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.lib.gy_return_map_counts(self.h, C.byref(a), C.byref(b), int(reset)))
+        return {"network": a.value, "fallback": b.value}
+
     def close(self):
         if self.h:
             self.lib.gy_destroy(self.h)
@@ -332,9 +344,9 @@ def _f32(a):
 
 
 class Mlp:
-    """The return-map network on the device (gy_mlp_*): 13 -> 64 -> 64 -> 64 -> 7, SiLU,
+    """The return-map network on the device (gy_mlp_*): 14 -> 64 -> 64 -> 64 -> 1, SiLU,
     BF16 matrix-core products with FP32 accumulation."""
-    N_IN, N_OUT = 13, 7
+    N_IN, N_OUT = 14, 1
 
     def __init__(self, seed: int = 0):
         self.lib = load()
@@ -361,10 +373,11 @@ class Mlp:
         assert p.size == self.nparam
         self._check(self.lib.gy_mlp_set_params(self.h, p.ctypes.data_as(_FP)))
 
-    def set_norm(self, in_mean, in_std, in_lo, in_hi, out_scale):
-        a = [_f32(x) for x in (in_mean, in_std, in_lo, in_hi, out_scale)]
-        self._norm = a
-        self._check(self.lib.gy_mlp_set_norm(self.h, *[x.ctypes.data_as(_FP) for x in a]))
+    def set_norm(self, in_mean, in_std, in_lo, in_hi, out_scale=1.0, out_offset=0.0, out_lo=-1e30, out_hi=1e30):
+        a = [_f32(x) for x in (in_mean, in_std, in_lo, in_hi)]
+        self._norm = a + [float(out_scale), float(out_offset), float(out_lo), float(out_hi)]
+        self._check(self.lib.gy_mlp_set_norm(self.h, *[x.ctypes.data_as(_FP) for x in a], float(out_scale),
+                                             float(out_offset), float(out_lo), float(out_hi)))
 
     def forward(self, x) -> np.ndarray:
         x = _f32(x)
@@ -399,18 +412,25 @@ class Mlp:
             pass
 
 
-def dp_map(mat, samples, net: "Mlp | None" = None) -> dict:
-    """The Drucker-Prager return map over raw samples [n, 13] on the GPU: exact FP32 map
-    (net None) or the gated surrogate. Returns out [n, 7], counts and the kernel time."""
+DP_CLOSED, DP_REFERENCE, DP_LEARNED = 0, 1, 2   # GY_DP_RETURN_*
+
+
+def dp_map(mat, samples, net: "Mlp | None" = None, mode: "int | None" = None) -> dict:
+    """The Drucker-Prager return map over raw samples [n, 13] on the GPU (gy_dp_map): the
+    closed form (default), the reference's loop (mode DP_REFERENCE) or, given `net`, the
+    gated network. Returns out [n, 7], the cone multipliers, counts and the kernel time."""
     lib = load()
+    if mode is None:
+        mode = DP_LEARNED if net is not None else DP_CLOSED
     x = _f32(samples)
     y = np.zeros((x.shape[0], 7), np.float32)
+    mult = np.zeros(x.shape[0], np.float32)
     nn_, nf, ms = C.c_int64(), C.c_int64(), C.c_double()
-    rc = lib.gy_dp_map(C.byref(mat), net.h if net else None, 1 if net else 0, x.ctypes.data_as(_FP),
-                       y.ctypes.data_as(_FP), x.shape[0], C.byref(nn_), C.byref(nf), C.byref(ms))
+    rc = lib.gy_dp_map(C.byref(mat), net.h if net else None, mode, x.ctypes.data_as(_FP), y.ctypes.data_as(_FP),
+                       mult.ctypes.data_as(_FP), x.shape[0], C.byref(nn_), C.byref(nf), C.byref(ms))
     if rc != GY_OK:
         raise GyError(rc, lib.gy_mlp_last_error(net.h).decode() if net else "gy_dp_map failed")
-    return {"out": y, "network": nn_.value, "fallback": nf.value, "ms": ms.value}
+    return {"out": y, "multiplier": mult, "network": nn_.value, "fallback": nf.value, "ms": ms.value}
 
 
 def counting_sort(keys, nkeys: int):

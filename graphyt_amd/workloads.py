@@ -138,7 +138,7 @@ def elastic_cube(h=0.02, L=0.5, side=0.2, steps=500) -> Scene:
 
 def dp_column(h=0.01, L=(2.0, 1.0, 1.0), lo=(0.03, 0.03, 0.30), size=(0.4, 0.8, 0.4),
               steps=1000, jitter=0.05, cohesion=0.0, phi=30.0, psi=0.0, seed=SEED,
-              apex_guard=True) -> Scene:
+              apex_guard=True, dp_return=0) -> Scene:
     """Drucker-Prager granular column collapse (model 4, rho 2000, K 5e7, G 2e7).
 
     Floor nodes (y <= 2h) get grid_vx_vy [0, 0]; wall nodes (x <= 2h or x >= L-2h) get
@@ -148,12 +148,13 @@ def dp_column(h=0.01, L=(2.0, 1.0, 1.0), lo=(0.03, 0.03, 0.30), size=(0.4, 0.8, 
     With c = 0 and psi = 0 the reference's apex return evaluates 0*inf (NaN) for every
     yielding particle in tension (ConstitutiveModels.cpp:364-365); `apex_guard` (default on)
     selects the guarded apex that returns sigma = 0 there, in the oracle and on the GPU alike.
+    `dp_return` picks the multipliers of the return map (gy_material::dp_return).
     """
     psep = h / 2
     counts = tuple(int(round(size[k] / psep)) for k in range(3))
     pos = lattice(lo, counts, psep, jitter=jitter, seed=seed)
     m = material(model=4, bulk_mod=5e7, shear_mod=2e7, density=2000.0, cohesion=cohesion,
-                 phi=phi, psi=psi, critical_strain=0.01, dp_apex_guard=apex_guard)
+                 phi=phi, psi=psi, critical_strain=0.01, dp_apex_guard=apex_guard, dp_return=dp_return)
     s = Scene("dp_column", tuple(L), h, psep, [m], gravity=(0, -9.81, 0), steps=steps)
     s.particles = _particles(pos)
     npos = s.node_positions()
@@ -179,6 +180,16 @@ def dp_column_mid(steps=1000, **kw) -> Scene:
     enough for the FP64 oracle to run 1000 steps in seconds."""
     return dp_column(h=0.01, L=(0.6, 0.5, 0.3), lo=(0.03, 0.03, 0.1), size=(0.1, 0.2, 0.1),
                      steps=steps, **kw)
+
+
+def dp_column_4m(steps=1000, **kw) -> Scene:
+    """The Drucker-Prager column at h = 0.00625 (psep = 0.003125): the same 0.4x0.8x0.4 m
+    column and 2x1x1 m domain, 128x256x128 = 4,194,304 particles. As in dp_column at
+    h = 0.01 (lo = 3h, 3h), the column starts 3h from the x and y origin, so the GIMP
+    support of its lowest particles reaches the floor nodes (y <= 2h) at once; with
+    lo = 0.03 it would fall 1.1 cm (about 12,000 steps) before touching them."""
+    h = 0.00625
+    return dp_column(h=h, lo=(3 * h, 3 * h, 0.30), steps=steps, **kw)
 
 
 def exact_length(n_nodes: int, h: float) -> float:

@@ -90,7 +90,12 @@ typedef struct gy_material {
    * cohesionless, non-dilatant material (c = 0, psi = 0) returns to sigma = 0 instead of
    * the reference's 0*inf = NaN. 0 = reference behaviour. */
   int32_t dp_apex_guard;
-  int32_t reserved;
+  /* Drucker-Prager multipliers (not in the reference; model 4 only, 0 for others):
+   * GY_DP_RETURN_CLOSED (0) Newton to its fixed point, GY_DP_RETURN_REFERENCE (1) the
+   * reference's loop control (ConstitutiveModels.cpp:321-343, 353-374: up to 201
+   * passes), GY_DP_RETURN_LEARNED (2) the network of gy_set_return_network (3D
+   * single-body contexts, is_damage = 0). Modes 1 and 2 run in 3D single-body contexts. */
+  int32_t dp_return;
 } gy_material;
 
 /* Parameters (Parameters.hpp:16-32) plus the particle spacing MaterialPoints::psep and
@@ -275,39 +280,64 @@ int  gy_set_tuning(gy_ctx *ctx, const gy_tuning *t);
 int  gy_autotune(gy_ctx *ctx, int32_t trial_steps, int64_t *best_persist_wgs, double *best_ms_per_step);
 
 /* ---- learned Drucker-Prager return map -------------------------------------------
- * No counterpart in the reference. A 13-64-64-64-7 SiLU network (BF16 matrix-core
- * products, FP32 accumulation) trained on the GPU on samples of the return map
- * (druckerPrager_Damage, ConstitutiveModels.cpp:238-428). Sample layout: inputs
- * [strain increment xx yy zz xy yz xz, prior stress (same order), accumulated plastic
- * strain], outputs [new stress, new accumulated plastic strain]. The network works on
- * normalised values: inputs (x - mean) / std, outputs the plastic correction of the
- * elastic trial stress and the plastic-strain increment, each divided by out_scale. */
-#define GY_MLP_IN      13
-#define GY_MLP_OUT     7
+ * No counterpart in the reference. A 14-64-64-64-1 SiLU network (BF16 matrix-core
+ * products, FP32 accumulation), trained on the GPU on samples of the return map
+ * (druckerPrager_Damage, ConstitutiveModels.cpp:238-428), gives the multiplier of the
+ * cone return; the new stress (6) and accumulated plastic strain (1) are rebuilt from it
+ * along the exact return direction, and the apex return is exact.
+ * Constitutive sample (GY_DP_IN floats): strain increment xx yy zz xy yz xz, prior stress
+ * (same order), accumulated plastic strain. Map output (GY_DP_OUT floats): new stress,
+ * new accumulated plastic strain.
+ * Network input (GY_MLP_IN floats): the sample, then the material's friction angle in
+ * degrees, each normalised as (x - mean) / std. Network output (GY_MLP_OUT): y, with the
+ * cone multiplier dL = (out_scale * y + out_offset) * F / (G + K eta eta_bar), F the
+ * trial yield function: the multiplier relative to that of a cone without hardening. */
+#define GY_DP_IN       13
+#define GY_DP_OUT      7
+#define GY_MLP_IN      14
+#define GY_MLP_OUT     1
 #define GY_MLP_PARAMS  10448   /* padded layout: W1 16x64, W2 64x64, W3 64x64, W4 64x16, biases */
 typedef struct gy_mlp gy_mlp;
 int  gy_mlp_create(gy_mlp **out, uint64_t seed);
 int  gy_mlp_param_count(void);
 int  gy_mlp_set_params(gy_mlp *m, const float *params);
 int  gy_mlp_get_params(gy_mlp *m, float *params);
-/* normalisation, and the per-input range [lo, hi] (normalised units) of the gate */
+/* normalisation (GY_MLP_IN each), the per-input range [lo, hi] (normalised units) of the
+ * gate, the output's scale and offset, and the range [out_lo, out_hi] of the scaled output
+ * the gate accepts (outside it, or where the return's apex/cone branch would differ
+ * across it, the sample takes the closed form) */
 int  gy_mlp_set_norm(gy_mlp *m, const float *in_mean, const float *in_std, const float *in_lo,
-                     const float *in_hi, const float *out_scale);
-/* network on normalised inputs x[n][13] -> y[n][7] */
+                     const float *in_hi, float out_scale, float out_offset, float out_lo, float out_hi);
+/* network on normalised inputs x[n][GY_MLP_IN] -> y[n][GY_MLP_OUT] */
 int  gy_mlp_forward(gy_mlp *m, const float *x, int64_t n, float *y);
 /* gradient of mean((y - t)^2) over the n samples w.r.t. the GY_MLP_PARAMS parameters */
 int  gy_mlp_gradient(gy_mlp *m, const float *x, const float *t, int64_t n, float *grad, double *loss);
 /* training set (normalised) kept on the device; Adam over mini-batches */
 int  gy_mlp_load_data(gy_mlp *m, const float *x, const float *t, int64_t n);
 int  gy_mlp_train(gy_mlp *m, int32_t epochs, int32_t batch, double lr, double lr_decay, double *last_loss);
-/* The return map over n raw samples: exact FP32 map (surrogate = 0), or the gated
- * surrogate (elastic trial exactly; yielding samples in the trained range through the
- * network, the others through the exact map). Reports how many samples took the network
- * and the fallback, and the kernel time. */
-int  gy_dp_map(const gy_material *mat, gy_mlp *m, int32_t surrogate, const float *in, float *out, int64_t n,
-               int64_t *n_network, int64_t *n_fallback, double *kernel_ms);
+/* Return-map modes (gy_dp_map and gy_material::dp_return). */
+#define GY_DP_RETURN_CLOSED     0
+#define GY_DP_RETURN_REFERENCE  1
+#define GY_DP_RETURN_LEARNED    2
+/* The return map over n raw samples of material mat (model 4; strain increments, so
+ * dt = 1): in[n][GY_DP_IN] -> out[n][GY_DP_OUT]. multiplier (optional, n floats): the
+ * cone multiplier each sample used, 0 for elastic samples. Mode GY_DP_RETURN_LEARNED
+ * needs m: elastic trial states exactly, yielding samples through the network when the
+ * gate accepts them (gy_mlp_set_norm), the others in closed form; n_network and
+ * n_fallback count them.
+ * kernel_ms: the kernel's device time. */
+int  gy_dp_map(const gy_material *mat, gy_mlp *m, int32_t mode, const float *in, float *out, float *multiplier,
+               int64_t n, int64_t *n_network, int64_t *n_fallback, double *kernel_ms);
 const char *gy_mlp_last_error(const gy_mlp *m);
 void gy_mlp_destroy(gy_mlp *m);
+
+/* The network of the materials with dp_return = GY_DP_RETURN_LEARNED, in the step: a copy
+ * of m's parameters and normalisation as they are now. Needed before gy_iterate when
+ * particles of such a material are uploaded. No counterpart in the reference. */
+int  gy_set_return_network(gy_ctx *ctx, gy_mlp *m);
+/* Yielding samples of learned materials that took the network, and those the gate sent
+ * to the closed form, since the upload or the last reset. */
+int  gy_return_map_counts(gy_ctx *ctx, int64_t *network, int64_t *fallback, int32_t reset);
 
 /* Options: see GY_OPT_*. */
 #define GY_OPT_USE_GRAPH   1   /* replay gy_iterate_n through a captured hipGraph (default 1) */

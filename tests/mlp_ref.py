@@ -1,6 +1,6 @@
 """CPU checker for the return-map network (test infrastructure only).
 
-A numpy restatement of the network the GPU kernels evaluate: 13 -> 64 -> 64 -> 64 -> 7,
+A numpy restatement of the network the GPU kernels evaluate: 14 -> 64 -> 64 -> 64 -> 1,
 SiLU, parameters in the padded flat layout of gy_mlp (W1 16x64, W2 64x64, W3 64x64,
 W4 64x16, row-major [k][n], then b1, b2, b3, b4). With `bf16=True` it rounds to
 bfloat16 (round to nearest even) exactly where the GPU kernels feed the matrix cores:
@@ -10,7 +10,7 @@ carried in float64 here, float32 on the GPU.
 """
 import numpy as np
 
-NIN, NOUT, KIN, HID, KOUT = 13, 7, 16, 64, 16
+NIN, NOUT, KIN, HID, KOUT = 14, 1, 16, 64, 16
 SHAPES = [(KIN, HID), (HID, HID), (HID, HID), (HID, KOUT)]
 NW = sum(a * b for a, b in SHAPES)
 NPARAM = NW + 3 * HID + KOUT
