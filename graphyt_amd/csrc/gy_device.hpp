@@ -23,6 +23,18 @@ __device__ __forceinline__ uint32_t u32(float f) { return __float_as_uint(f); }
 // particle kernels (measured on the 1M-particle column: fused kernel -2 %, grid kernel
 // -12 %, step -3 %); the plane kernel reads plain (there streaming measured 6 % slower).
 __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p) { return __builtin_nontemporal_load(p); }
+// A load through the constant address space, for tables no kernel of the step writes while
+// the reading kernel runs (instance and tile tables, offsets, the active tile list): at a
+// workgroup-uniform address it becomes a scalar load, which keeps it out of the in-order
+// vector-memory counter that the vector loads a kernel issues ahead of use are waited on.
+template <class T>
+__device__ __forceinline__ T cld(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *reinterpret_cast<const __attribute__((address_space(4))) T *>(reinterpret_cast<uintptr_t>(p));
+#else
+  return *p;  // the host pass parses device functions but never runs them
+#endif
+}
 
 // Extended position and velocity (the F_XV words). A position is the 48-bit fixed-point
 // number (q << 16) + lo in units of 2^-16 quantum; a step advances it by d quanta with one
@@ -202,8 +214,8 @@ __device__ __forceinline__ int hist_rank(int base, int hl) {
   return __shfl(base, hl, 64) + (int)(threadIdx.x & 63) - hl;
 }
 
-// The particle kernels' work queue over the active tiles: heavy tiles first (in tile
-// order), then the light ones, so the claims that end a launch are the short ones; one
+// The particle kernels' work queue over the active tiles, longest first (entry-count
+// classes), so the claims that end a launch are the short ones; one
 // counter for the whole device (balance across the XCDs matters more here than keeping
 // neighbouring tiles in one L2: measured). Bounds read once per launch.
 struct TileQueue {

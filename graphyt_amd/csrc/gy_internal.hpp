@@ -144,7 +144,7 @@ struct Ctrl {
   int oor;           // count of out-of-range particles (clamped)
   int epoch;         // tile partials written with tile_epoch == epoch are this step's P2G
   int active_tiles;  // tiles holding particles in the current binning
-  int heavy_tiles;   // active tiles holding more than one chunk of entries
+  int heavy_tiles;   // leading active-list entries claimed in list order (all: the list is longest first)
   // work counter of the persistent particle kernels, on a line of its own: every
   // workgroup adds to it, and loads of fields sharing its line stall behind those atomics
   alignas(256) int next;
@@ -164,7 +164,7 @@ struct DevPtrs {
   int *offsets;          // per-key exclusive offsets (nkeys + 1)
   int *blocksum;         // scan partials (particle counts per scan block)
   int *blocktiles;       // scan partials (active tiles per scan block)
-  int *active;           // active tiles: heavy ones ascending from the front, light ones from the back
+  int *active;           // active tiles, longest first: entry-count classes, each in ascending tile order
   int *tile_epoch;       // per tile: epoch of its last written core partials
   int *halo_epoch;       // per tile: epoch of its last written halo-shell partials
   uint64_t *nodemask;    // per tile: 8 words, bit l set when region node l has mass

@@ -5,15 +5,15 @@
 //               the P2G tile partials of every node of the periodic class, v = p/m,
 //               a = f/m + g, body-1 kick (half at step 1), BCs, fan-out to every mirror
 //               node (Grid.cpp:9-245, 310-603, 862-1094)
-//   k_g2p2g     persistent over the active cell tiles (one work queue, tiles with more
-//               than one chunk of particles first): G2P from LDS-staged node
+//   k_g2p2g     persistent over the active cell tiles (one work queue, tiles with the most
+//               particles first): G2P from LDS-staged node
 //               velocities, particle update, constitutive/damage models, periodic wrap,
 //               new cell key + histogram, and P2G of the updated particle into an FP64 LDS
 //               region with a one-node halo, written out as per-tile FP32 partial sums
 //               with plain stores (Extrapolation.cpp:31-367, TimeIntegration.cpp:11-296,
 //               ConstitutiveModels.cpp, DamageModels.cpp)
 //   k_scan_*    exclusive scan of the per-cell histogram (counting sort offsets), the
-//               active tile list (heavy tiles from the front, light from the back) and
+//               active tile list (longest first, by entry-count class) and
 //               the flagged-brick list; the first also does the step/parity/time
 //               bookkeeping
 //   k_place     cell-grouped slot list (within a cell in histogram order; the stable
@@ -129,6 +129,17 @@ __device__ __forceinline__ void load16(const int *p, int c[16]) {
   }
 }
 
+// Size class of a tile holding n entries, for the particle kernels' claim order (longest
+// first): more than two chunks, two, more than half of one, the rest; -1 when empty.
+constexpr int NBUCKET = 4;
+__device__ __forceinline__ int tile_bucket(int n) {
+  if (n <= 0) return -1;
+  if (n > 2 * CHUNK_ENTRIES) return 0;
+  if (n > CHUNK_ENTRIES) return 1;
+  if (n > CHUNK_ENTRIES / 2) return 2;
+  return 3;
+}
+
 // With `advance`, block 0 also performs the step bookkeeping: times, step, parity and
 // epoch (no block of this kernel reads them; the kernels after it see the new values).
 // Append the flagged bricks among gb, gb + stride, ... to the brick list: one atomic per
@@ -173,33 +184,35 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
   }
   int ts = s + __shfl_xor(s, 1, 64);
   ts += __shfl_xor(ts, 2, 64);
-  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
-  int v3[3] = {s, act, (act && ts > CHUNK_ENTRIES) ? 1 : 0}, t3[3];
-  block_scan_n(v3, t3);
-  const int tot = t3[0], tact = t3[1], theavy = t3[2];
+  const int bk = ((threadIdx.x & 3) == 0) ? tile_bucket(ts) : -1;
+  int v5[5] = {s, bk == 0, bk == 1, bk == 2, bk == 3}, t5[5];
+  block_scan_n(v5, t5);
   if (threadIdx.x == 0) {
-    d.blocksum[blockIdx.x] = tot;
-    d.blocktiles[blockIdx.x] = tact | (theavy << 16);  // at most SCAN_BLOCK / TILE3 each
+    d.blocksum[blockIdx.x] = t5[0];
+    // tiles per bucket, at most SCAN_BLOCK / TILE3 = 64 each: one byte each
+    d.blocktiles[blockIdx.x] = t5[1] | (t5[2] << 8) | (t5[3] << 16) | (t5[4] << 24);
   }
 }
 
 __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
-  // prefixes of all earlier blocks (particles and active tiles), and the totals
-  int pre = 0, pret = 0, preh = 0, allt = 0, allh = 0;
+  // prefixes of all earlier blocks (particles and active tiles per bucket), and the totals
+  int pre = 0, preb[NBUCKET] = {0, 0, 0, 0}, allb[NBUCKET] = {0, 0, 0, 0};
   for (int j = threadIdx.x; j < (int)gridDim.x; j += WG) {
-    const int bt = d.blocktiles[j] & 0xffff, bh = d.blocktiles[j] >> 16;
-    if (j < (int)blockIdx.x) {
-      pre += d.blocksum[j];
-      pret += bt;
-      preh += bh;
+    const int bt = d.blocktiles[j];
+#pragma unroll
+    for (int b = 0; b < NBUCKET; b++) {
+      const int nb = (bt >> (8 * b)) & 0xff;
+      if (j < (int)blockIdx.x) preb[b] += nb;
+      allb[b] += nb;
     }
-    allt += bt;
-    allh += bh;
+    if (j < (int)blockIdx.x) pre += d.blocksum[j];
   }
   {
-    int v5[5] = {pre, pret, preh, allt, allh}, t5[5];
-    block_scan_n(v5, t5);
-    pre = t5[0]; pret = t5[1]; preh = t5[2]; allt = t5[3]; allh = t5[4];
+    int v9[9] = {pre, preb[0], preb[1], preb[2], preb[3], allb[0], allb[1], allb[2], allb[3]}, t9[9];
+    block_scan_n(v9, t9);
+    pre = t9[0];
+#pragma unroll
+    for (int b = 0; b < NBUCKET; b++) { preb[b] = t9[1 + b]; allb[b] = t9[5 + b]; }
   }
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   const bool have = base < d.nkeys;
@@ -211,15 +224,14 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     if (!have) c[j] = 0;
     s += c[j];
   }
-  // exclusive in-block prefixes of the particle count and of the active and heavy tiles
+  // exclusive in-block prefixes of the particle count and of each bucket's tiles
   int ts = s + __shfl_xor(s, 1, 64);
   ts += __shfl_xor(ts, 2, 64);
-  const int act = ((threadIdx.x & 3) == 0 && ts > 0) ? 1 : 0;
-  const int heavy = (act && ts > CHUNK_ENTRIES) ? 1 : 0;
-  int v3[3] = {s, act, heavy}, t3[3];
-  block_scan_n(v3, t3);
-  const int tot = t3[0], ttot = t3[1];
-  int run = pre + v3[0];
+  const int bk = ((threadIdx.x & 3) == 0) ? tile_bucket(ts) : -1;
+  int v5[5] = {s, bk == 0, bk == 1, bk == 2, bk == 3}, t5[5];
+  block_scan_n(v5, t5);
+  const int tot = t5[0];
+  int run = pre + v5[0];
   if (have) {
     int4 *ov = reinterpret_cast<int4 *>(d.offsets + base);
     int4 *cv = reinterpret_cast<int4 *>(d.count + base);
@@ -235,15 +247,22 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
       if ((c[4 * q] | c[4 * q + 1] | c[4 * q + 2] | c[4 * q + 3]) != 0) cv[q] = make_int4(0, 0, 0, 0);
     }
   }
-  // compacted active tile list: heavy tiles (more than one chunk of entries) ascending
-  // from the front, light ones ascending from the back
-  const int pos = pret + v3[1];   // index in tile order
-  const int hpos = preh + v3[2];  // heavy tiles before it
-  if (act) d.active[heavy ? hpos : allt - 1 - (pos - hpos)] = (int)(base / TILE3);
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allh;
+  // active tile list, longest first: the buckets in order (most entries first), each in
+  // ascending tile order; claim k of the particle kernels takes entry k
+  if (bk >= 0) {
+    int start = 0, mine = 0;
+#pragma unroll
+    for (int b = 0; b < NBUCKET; b++) {
+      if (b < bk) start += allb[b];
+      if (b == bk) mine = preb[b] + v5[1 + b];
+    }
+    d.active[start + mine] = (int)(base / TILE3);
+  }
+  const int allt = allb[0] + allb[1] + allb[2] + allb[3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allt;  // the list is in claim order
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     d.offsets[d.nkeys] = pre + tot;
-    d.ctrl->active_tiles = pret + ttot;
+    d.ctrl->active_tiles = allt;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   // list of the bricks the particle kernel flagged, for the next grid update
@@ -538,6 +557,77 @@ __device__ __forceinline__ void dp_learned_gate(const float *norm, const MatDev 
   dp_return<false>(M, T, dL, s, p, epb);
 }
 
+// Tile of claim k over the active list (longest first), -1 past the end; scalar loads.
+__device__ __forceinline__ int claimed_tile(const DevPtrs &d, int nh, int n, int k) {
+  if (k < nh) return cld(&d.active[k]);
+  if (k < n) return cld(&d.active[n - 1 - (k - nh)]);
+  return -1;
+}
+
+// Instance and grid placement of tile t (scalar loads).
+struct TileGeo {
+  int ii;               // instance
+  int o[3];             // first cell (= first node of the G2P region) per axis
+  int n[3];             // nodes per axis
+  long long node_base;  // first global node of the instance
+};
+__device__ __forceinline__ TileGeo tile_geo(const DevPtrs &d, int t) {
+  TileGeo g;
+  g.ii = d.ninst == 1 ? 0 : cld(&d.tile_inst[t]);
+  const InstDesc *Ip = &d.inst[g.ii];
+  const int lt = t - cld(&Ip->tile_base), nt0 = cld(&Ip->nt[0]), nt1 = cld(&Ip->nt[1]);
+  g.o[0] = (lt % nt0) * TILE;
+  g.o[1] = ((lt / nt0) % nt1) * TILE;
+  g.o[2] = (lt / (nt0 * nt1)) * TILE;
+#pragma unroll
+  for (int a = 0; a < 3; a++) g.n[a] = cld(&Ip->n[a]);
+  g.node_base = cld(&Ip->node_base);
+  return g;
+}
+
+// The vector loads a tile starts with: thread tid's entry of its 65 offsets and (VEL) the
+// velocity record of node tid of its G2P region, every lane loading a valid address (an
+// out-of-grid node reads the instance's first node and gets keep = 0). Nothing here uses
+// the values, so nothing waits for them here.
+template <bool VEL>
+__device__ __forceinline__ void tile_loads(const DevPtrs &d, int t, const TileGeo &g, int tid, int &off,
+                                           float4 &v0, float4 &v1, uint32_t &keep) {
+  off = d.offsets[t * TILE3 + min(tid, TILE3)];
+  if constexpr (VEL) {
+    const int n = min(tid, NODES_G2P - 1);
+    const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
+    const int g0 = g.o[0] + x, g1 = g.o[1] + y, g2 = g.o[2] + z;
+    const bool in = tid < NODES_G2P && g0 < g.n[0] && g1 < g.n[1] && g2 < g.n[2];
+    const long long ni = g.node_base + (in ? g0 + (long long)g.n[0] * (g1 + (long long)g.n[1] * g2) : 0);
+    const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
+    v0 = vp[0];
+    v1 = vp[1];
+    keep = in ? 0xffffffffu : 0u;
+  }
+}
+
+// ... into the places the tile's first chunk reads: s_off (tid <= TILE3) and (VEL) the raw
+// node records, made relative to the tile's reference velocity when the tile starts.
+// Out-of-grid records are zeroed by masking their bits (a select of the values lets the
+// compiler select the address instead: a generic load of a stack zero).
+template <bool VEL>
+__device__ __forceinline__ void stage_ahead(int tid, int off, float4 a, float4 b, uint32_t m, int *s_off,
+                                            float4 (*s_vel)[2]) {
+  if (tid <= TILE3) s_off[tid] = off;
+  if constexpr (VEL) {
+    if (tid < NODES_G2P) {
+      s_vel[tid][0] = make_float4(f32(u32(a.x) & m), f32(u32(a.y) & m), f32(u32(a.z) & m), f32(u32(a.w) & m));
+      s_vel[tid][1] = make_float4(f32(u32(b.x) & m), f32(u32(b.y) & m), f32(u32(b.z) & m), f32(u32(b.w) & m));
+    }
+  }
+}
+
+// `publish` runs in an update after its G2P and before its first store (the fused kernel
+// hands the next tile's claim over there).
+struct NoPublish {
+  __device__ void operator()() const {}
+};
+
 // Phase A for one list entry i (slot `slot` of the source buffer): G2P + update +
 // constitutive + store at sorted position i (UPDATE), and the P2G record of the (new)
 // state into column j of s_rec. Returns the new cell key (UPDATE) or -1.
@@ -547,13 +637,14 @@ __device__ __forceinline__ void dp_learned_gate(const float *norm, const MatDev 
 // MB (two-body contact): G2P reads the velocity field of the body slot the particle's
 // object holds at each node (s_vel: body 1, s_vel2: body 2, s_obj: the slots' objects;
 // Extrapolation.cpp:288-367). The record is written when s_rec is given.
-template <bool UPDATE, int MASK, bool MB = false>
+template <bool UPDATE, int MASK, bool MB = false, class Publish = NoPublish>
 __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc &I, const int o[3], double tl,
                                                int i, int slot, const float4 (*s_vel)[2], float3 vref,
                                                const MatDev *s_mat,
                                                int par, RecLds *s_rec, int j,
                                                const float4 (*s_vel2)[2] = nullptr,
-                                               int (*s_obj)[NODES_G2P] = nullptr, DpPend *pd = nullptr) {
+                                               int (*s_obj)[NODES_G2P] = nullptr, DpPend *pd = nullptr,
+                                               const Publish &publish = Publish()) {
   constexpr bool DPNN = (MASK & KM_DPNN) != 0, DPLOOP = (MASK & KM_DPLOOP) != 0;
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
   uint32_t *B = (par ? d.pbuf[0] : d.pbuf[1]);
@@ -649,6 +740,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
     er[4] = 0.5f * (Ly[2] + Lz[1]);
     er[5] = 0.5f * (Lx[2] + Lz[0]);
     const float rho_dot = -rho * (Lx[0] + Ly[1] + Lz[2]);
+    publish();
 
     const int bct = (MASK & KM_PBC) ? tag_bct(tag) : 0;
     float bv[4] = {0, 0, 0, 0};
@@ -969,18 +1061,27 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
   const int tid = threadIdx.x;
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
-  const TileQueue queue(d);
+  const int nh = cld(&d.ctrl->heavy_tiles), na = cld(&d.ctrl->active_tiles);
   unsigned nn_net = 0, nn_fb = 0;  // this wave's network and fallback samples
   if constexpr (DPNN) nn_stage(d.nn, s_nnw, s_nnb);  // visible after the loop's first barrier
 
+  // A tile's successor is claimed when its last chunk starts (one work counter, longest
+  // tiles first); the successor's offsets and node velocities are loaded during that
+  // chunk's scatter and put in place after it, so a tile starts with no load round trip.
+  constexpr bool VEL = UPDATE && !MB;
+  int off_r = 0;
+  float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
+  uint32_t vkeep = 0;
+  if (tid == 0) s_tile = atomicAdd(&d.ctrl->next, 1);
+  __syncthreads();
+  int t = claimed_tile(d, nh, na, __builtin_amdgcn_readfirstlane(s_tile));
+  if (t >= 0) {
+    tile_loads<VEL>(d, t, tile_geo(d, t), tid, off_r, va, vb, vkeep);
+    stage_ahead<VEL>(tid, off_r, va, vb, vkeep, s_off, s_vel);
+  }
+  int ii_mat = -1;  // instance whose materials s_mat holds
   for (;;) {
-    if (tid == 0) {
-      s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
-    }
-    __syncthreads();
-    // workgroup-uniform values read from LDS: keep them (and all that derives from
-    // them, the instance description included) in scalar registers
-    const int t = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();  // the staged offsets and node records are in place
     if (t < 0) {
       if (DPNN && (tid & 63) == 0 && (nn_net | nn_fb)) {
         atomicAdd(&d.ctrl->nn_count[0], (unsigned long long)nn_net);
@@ -988,19 +1089,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       }
       break;
     }
-    const int k0 = t * TILE3;
-    const int ii = d.tile_inst[t];
-    const InstDesc I = d.inst[ii];
-    const int lt = t - I.tile_base;
-    const int o[3] = {(lt % I.nt[0]) * TILE, ((lt / I.nt[0]) % I.nt[1]) * TILE, (lt / (I.nt[0] * I.nt[1])) * TILE};
+    // workgroup-uniform values (scalar loads): keep them, the instance description
+    // included, in scalar registers
+    const TileGeo g = tile_geo(d, t);
+    const int ii = g.ii;
+    const InstDesc I = cld(&d.inst[ii]);
+    const int o[3] = {g.o[0], g.o[1], g.o[2]};
     float3 vref = make_float3(0.f, 0.f, 0.f);  // staged node velocities are relative to vref
 
     {
       double2 *z = reinterpret_cast<double2 *>(&s_acc[0][0]);
       for (int n = tid; n < 7 * ACC_N / 2; n += WG) z[n] = make_double2(0.0, 0.0);
     }
-    if (tid <= TILE3) s_off[tid] = d.offsets[k0 + tid];
-    if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+    if (ii != ii_mat) {  // workgroup-uniform; a single-instance context loads them once
+      if (tid < I.nmat && tid < MAXMAT) s_mat[tid] = d.mats[I.mat_base + tid];
+      ii_mat = ii;
+    }
     if constexpr (MB) {
       for (int n = tid; n < NODES_G2P; n += WG) stage_body_vel(d, I, o, n, s_vel, s_vel2, s_obj);
     } else if (UPDATE) {
@@ -1018,13 +1122,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       int key = 0x7fffffff;
       if (n < NODES_G2P) {
         const int x = n % G2P_R, y = (n / G2P_R) % G2P_R, z = n / (G2P_R * G2P_R);
-        const int g0 = o[0] + x, g1 = o[1] + y, g2 = o[2] + z;
-        if (g0 < I.n[0] && g1 < I.n[1] && g2 < I.n[2]) {
-          const long long ni = I.node_base + g0 + (long long)I.n[0] * (g1 + (long long)I.n[1] * g2);
-          const float4 *vp = reinterpret_cast<const float4 *>(d.vel + ni * NODE_W);
-          v0 = vp[0];
-          v1 = vp[1];
-        }
+        v0 = s_vel[n][0];  // the raw record this thread staged ahead of the tile
+        v1 = s_vel[n][1];
         if (v1.z == 0.0f) {
           v0 = make_float4(0.f, 0.f, 0.f, 0.f);
           v1 = v0;
@@ -1041,7 +1140,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       int best = 0;
 #pragma unroll
       for (int w = 1; w < WG / 64; w++) best = (s_refkey[w] < s_refkey[best]) ? w : best;
-      const float4 r = (s_refkey[best] != 0x7fffffff) ? s_refv[best] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 r = s_refv[best];  // read unconditionally: a select of addresses is a generic load
+      if (s_refkey[best] == 0x7fffffff) r = make_float4(0.f, 0.f, 0.f, 0.f);
       vref = make_float3(r.x, r.y, r.z);
       if (n < NODES_G2P) {
         s_vel[n][0] = make_float4(v0.x - r.x, v0.y - r.y, v0.z - r.z, v0.w);
@@ -1050,9 +1150,28 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
     }
     __syncthreads();
 
-    const double tl = UPDATE ? d.t_last[ii] : 0.0;
+    const double tl = UPDATE ? cld(&d.t_last[ii]) : 0.0;
     const int p0 = __builtin_amdgcn_readfirstlane(s_off[0]), p1 = __builtin_amdgcn_readfirstlane(s_off[TILE3]);
+    int t_next = -1;
+    if (p0 >= p1) {  // a tile without entries (never in the active list)
+      __syncthreads();  // every thread has read p0 and p1
+      if (tid == 0) s_tile = atomicAdd(&d.ctrl->next, 1);
+      __syncthreads();
+      t_next = claimed_tile(d, nh, na, __builtin_amdgcn_readfirstlane(s_tile));
+      if (t_next >= 0) {
+        tile_loads<VEL>(d, t_next, tile_geo(d, t_next), tid, off_r, va, vb, vkeep);
+        stage_ahead<VEL>(tid, off_r, va, vb, vkeep, s_off, s_vel);
+      }
+    }
     for (int c0 = p0; c0 < p1; c0 += CHUNK) {
+      const bool last = c0 + CHUNK >= p1;  // workgroup-uniform
+      // the successor's claim, handed over in LDS after this update's G2P (before its stores,
+      // so nothing waits for the counter's round trip behind them)
+      int kclaim = 0;
+      if (last && tid == 0) kclaim = atomicAdd(&d.ctrl->next, 1);
+      const auto publish = [&] {
+        if (last && tid == 0) s_tile = kclaim;
+      };
       const int c1 = min(c0 + CHUNK, p1), nc = c1 - c0;
       // cell of this lane's entry: the last cell whose first entry is <= i (two rounds of
       // LDS reads over the tile's 65 offsets)
@@ -1094,11 +1213,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
 #pragma unroll
         for (int k = 0; k < 8; k++) pd.x[k] = 0u;
       }
-      if (i < c1)
+      if (i < c1) {
         knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, vref, s_mat, par, &s_rec, col,
                                                  MB ? s_vel2 : nullptr,
                                                  MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr,
-                                                 DPNN ? &pd : nullptr);
+                                                 DPNN ? &pd : nullptr, publish);
+        if constexpr (!UPDATE) publish();  // (no G2P in the P2G-only pass)
+      }
       if constexpr (DPNN) {
         // the wavefront's waiting samples through the network (all lanes take part), then
         // their return, stores and record
@@ -1117,12 +1238,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       int hl = 0, hbase = 0;
       if (UPDATE) hbase = hist_issue(d.count, knew, hl);
       __syncthreads();
+      // the successor's offsets and node records, in flight during the scatter
+      if (last) {
+        t_next = claimed_tile(d, nh, na, __builtin_amdgcn_readfirstlane(s_tile));
+        if (t_next >= 0) tile_loads<VEL>(d, t_next, tile_geo(d, t_next), tid, off_r, va, vb, vkeep);
+      }
       // phase B: column tid holds class-sorted entry (tid % 16) * P + tid / 16
       if ((tid >> 4) < P && (tid & 15) * P + (tid >> 4) < nc) particle_scatter(d, I, o, ii, &s_rec, tid, s_acc);
-      if (UPDATE) {
-        const int r = hist_rank(hbase, hl);
-        if (knew >= 0) d.rank[i] = r;
-      }
+      int r = 0;
+      if (UPDATE) r = hist_rank(hbase, hl);
+      // this tile's last reads of s_off and s_vel were in the update above
+      if (last && t_next >= 0) stage_ahead<VEL>(tid, off_r, va, vb, vkeep, s_off, s_vel);
+      if (UPDATE && knew >= 0) d.rank[i] = r;
       __syncthreads();
     }
 
@@ -1167,7 +1294,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
       if (tid == 0) d.halo_epoch[t] = epoch_w;
     }
     if (tid == 0) d.tile_epoch[t] = epoch_w;
-    __syncthreads();
+    t = t_next;
   }
 }
 

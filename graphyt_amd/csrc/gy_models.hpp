@@ -11,10 +11,6 @@ namespace {
 // ---------------------------------------------------------------------------------------
 // Constitutive models (FP32), following ConstitutiveModels.cpp operation by operation.
 
-struct PState {
-  float v[3], s[6], e[6], p[6], epb, rho, mass, dmg, smax;
-};
-
 // elastic, ConstitutiveModels.cpp:3-30
 __device__ __forceinline__ void cm_elastic(const MatDev &M, const float er[6], float s[6], float dt) {
   const float vol = 0.3333f * (er[0] + er[1] + er[2]);
