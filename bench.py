@@ -123,17 +123,25 @@ def run_cpu_baseline(cpu_steps: int, workload: str = "dp_column"):
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5) -> int:
-    """A plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters
-    per dispatch, divided by nbytes, give the scale applied to the fused kernel's counters."""
+    """Plain device copies of a known size for the HBM counters: a vectorised copy and a
+    gather (index_select with the identity index: 4-byte loads per lane, the access width
+    of the fused kernel's state loads). Each kernel's FETCH_SIZE / WRITE_SIZE per dispatch,
+    divided by its bytes, is the scale applied to the counters of kernels of that access
+    width."""
     import torch
     src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
+    idx = torch.arange(src.numel(), device="cuda", dtype=torch.int64)
     torch.cuda.synchronize()
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"copy calibration: {reps} copies of {nbytes} bytes")
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+    for _ in range(reps):
+        torch.index_select(src, 0, idx, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies and {reps} gathers of {nbytes} bytes (gather also reads "
+        f"{idx.numel() * 8} index bytes)")
+    print(json.dumps({"copy_bytes": nbytes, "gather_index_bytes": idx.numel() * 8, "reps": reps}))
     return 0
 
 
@@ -191,7 +199,6 @@ def bench_ensemble(args) -> int:
         "cpu_baseline": None,
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in stats.items()},
         "audit_nonfinite": audit["nonfinite"],
-        "surrogate": surrogate,
     }
     print(json.dumps(out))
     ctx.close()
@@ -380,7 +387,7 @@ def surrogate_config(args) -> dict:
         "speedup_vs_reference_loop": round(runs["reference_loop"]["ms_per_step"] / runs["learned"]["ms_per_step"], 4),
         "speedup_vs_closed_form": round(runs["closed_form"]["ms_per_step"] / runs["learned"]["ms_per_step"], 4),
         "rel_l2_vs_closed_form_after_steps": err,
-        "map": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in ev.items()},
+        "map": {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in ev.items()},
         "training": {"samples": n_samples, "phi_deg": [25, 30, 35, 40], "epochs": args.sur_epochs,
                      "collect_s": round(t_collect, 1), "train_s": round(t_fit, 1),
                      **{k: v for k, v in st.items() if k != "samples"}},
@@ -515,6 +522,7 @@ def main():
         "step_hbm_frac": round(B_ALG_STEP * best / (HBM_PEAK_GBS * 1e9), 4),
         "kernels_ms": {k: round(v[1] / v[0], 4) for k, v in stats.items()},
         "audit_nonfinite": audit["nonfinite"],
+        "surrogate": surrogate,
     }
     if args.profile_json:
         with open(args.profile_json, "w") as f:
