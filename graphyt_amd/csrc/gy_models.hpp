@@ -235,19 +235,21 @@ __device__ __forceinline__ float dp_apex_multiplier(const MatDev &M, const DpTri
 // stress of ~1e-31 Pa (the velocity gradient of a body at rest, to rounding) leaves J2
 // and F / G below the FP32 normal range, flushed to zero, and the cone return would
 // divide 0 by sqrt(J2) = 0.
+// Returns false when the apex multiplier is negative: s, p and epb are then left as they
+// were (the only case in which s is read).
 template <bool REF_LOOP>
-__device__ __forceinline__ void dp_return(const MatDev &M, const DpTrial &T, float dL, float s[6], float p[6],
+__device__ __forceinline__ bool dp_return(const MatDev &M, const DpTrial &T, float dL, float s[6], float p[6],
                                           float &epb) {
   const float G = M.G, K = M.K;
   if ((T.sJ2 - G * dL < 0.0f || T.sJ2 == 0.0f) && !M.damage) {
     const float de = dp_apex_multiplier<REF_LOOP>(M, T, epb);
-    if (de < 0.0f) return;  // deps < 0: stress left as it was (ConstitutiveModels.cpp:376-378)
+    if (de < 0.0f) return false;  // deps < 0: stress left as it was (ConstitutiveModels.cpp:376-378)
     const float sm = T.I1 - K * de;
     s[0] = sm; s[1] = sm; s[2] = sm; s[3] = 0.0f; s[4] = 0.0f; s[5] = 0.0f;
     epb += de * M.xi_over_eta;
     const float pv = de * (M.eta_bar / 3.0f);
     p[0] += pv; p[1] += pv; p[2] += pv;
-    return;
+    return true;
   }
   const float sxx = T.tr[0] - T.I1, syy = T.tr[1] - T.I1, szz = T.tr[2] - T.I1;
   const float r = 1.0f - G * dL / T.sJ2;
@@ -266,6 +268,7 @@ __device__ __forceinline__ void dp_return(const MatDev &M, const DpTrial &T, flo
   p[3] += dL * hj * T.tr[3];
   p[4] += dL * hj * T.tr[4];
   p[5] += dL * hj * T.tr[5];
+  return true;
 }
 
 // The cone multiplier of a learned material from the network's scaled output yr (the

@@ -95,6 +95,8 @@ def test_learned_step_against_plain_step_and_oracle(trained, oracle_mod, phi):
     assert counts["network"] > 100 * steps and counts["fallback"] <= 0.05 * counts["network"]
     for k in ("pos", "vel", "stress"):
         assert np.all(np.isfinite(got[k]))
-    assert vs_plain["stress"] <= 1e-3 and vs_oracle["stress"] <= 1e-3
-    assert vs_plain["vel"] <= 1e-3 and vs_oracle["vel"] <= 1e-3
+    # measured: stress 3.5e-5 (plain) and 4.4e-5 (oracle) at 30 and 35 degrees, as close as
+    # the plain GPU step itself is to the oracle (2.9e-5); held to the step's 1e-4 bar
+    assert vs_plain["stress"] <= 1e-4 and vs_oracle["stress"] <= 1e-4
+    assert vs_plain["vel"] <= 1e-4 and vs_oracle["vel"] <= 1e-4
     assert vs_plain["pos"] <= 1e-6 and vs_oracle["pos"] <= 1e-6
