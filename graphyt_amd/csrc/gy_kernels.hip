@@ -649,6 +649,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
                                                int par, RecLds *s_rec, int j,
                                                const float4 (*s_vel2)[2] = nullptr,
                                                int (*s_obj)[NODES_G2P] = nullptr, DpPend *pd = nullptr,
+                                               const float *s_nnn = nullptr,
                                                const Publish &publish = Publish()) {
   constexpr bool DPNN = (MASK & KM_DPNN) != 0, DPLOOP = (MASK & KM_DPLOOP) != 0;
   const uint32_t *A = (par ? d.pbuf[1] : d.pbuf[0]);
@@ -790,7 +791,7 @@ __device__ __forceinline__ int particle_update(const DevPtrs &d, const InstDesc 
         if ((MASK & KM_PERFECT) && M.model == 3) cm_perfect(M, er, s, p, epb, dt);
         if ((MASK & KM_DP) && M.model == 4) {
           if (DPNN && M.dp_mode == GY_DP_RETURN_LEARNED)
-            dp_learned_gate(d.nn.norm, M, tag_mat(tag), slot, er, s, p, epb, dt, *pd);
+            dp_learned_gate(s_nnn, M, tag_mat(tag), slot, er, s, p, epb, dt, *pd);
           else
             cm_drucker_prager<DPLOOP>(M, er, s, p, epb, dt);
         }
@@ -981,10 +982,9 @@ __device__ __forceinline__ void particle_scatter(const DevPtrs &d, const InstDes
 // direction (apex return exact), the stores of the stress and plastic fields at sorted
 // position out, and the stress terms of the record in column j. Returns whether the
 // network's multiplier was used.
-__device__ __forceinline__ bool learned_return(const DevPtrs &d, const MatDev *s_mat, const DpPend &pd, float y,
-                                               int par, int out, RecLds *s_rec, int j) {
+__device__ __forceinline__ bool learned_return(const DevPtrs &d, const MatDev *s_mat, const float *norm,
+                                               const DpPend &pd, float y, int par, int out, RecLds *s_rec, int j) {
   const MatDev &M = s_mat[pd.mat];
-  const float *norm = d.nn.norm;
   float dL = dp_learned_multiplier(M, pd.T, norm[NN_NORM_OUT] * y + norm[NN_NORM_OUT + 1], norm[NN_NORM_YLO],
                                    norm[NN_NORM_YHI]);
   const bool used = dL >= 0.0f;
@@ -1063,6 +1063,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
   __shared__ float4 s_refv[WG / 64];
   __shared__ uint2 s_nnw[DPNN ? NN_TILES * 64 : 1];               // network fragments
   __shared__ __align__(16) float s_nnb[DPNN ? NN_NBIAS : 4];       // network biases
+  __shared__ __align__(16) float s_nnn[DPNN ? NN_NORM_N : 4];      // network normalisation
   // a chunk's waiting samples, compacted: network inputs 0-3, 4-5 and 6 (input 7 is the
   // zero padding), and the network's output in place of input 6; per wave the count
   __shared__ uint4 s_nnx0[DPNN ? WG : 1];
@@ -1074,7 +1075,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
   const int nh = cld(&d.ctrl->heavy_tiles), na = cld(&d.ctrl->active_tiles);
   unsigned nn_net = 0, nn_fb = 0;  // this wave's network and fallback samples
-  if constexpr (DPNN) nn_stage(d.nn, s_nnw, s_nnb);  // visible after the loop's first barrier
+  if constexpr (DPNN) nn_stage(d.nn, s_nnw, s_nnb, s_nnn);  // visible after the loop's first barrier
 
   // A tile's successor is claimed when its last chunk starts (one work counter, longest
   // tiles first); the successor's offsets and node velocities are loaded during that
@@ -1228,7 +1229,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
         knew = particle_update<UPDATE, MASK, MB>(d, I, o, tl, i, d.tmp[i], s_vel, vref, s_mat, par, &s_rec, col,
                                                  MB ? s_vel2 : nullptr,
                                                  MB ? reinterpret_cast<int (*)[NODES_G2P]>(s_obj) : nullptr,
-                                                 DPNN ? &pd : nullptr, publish);
+                                                 DPNN ? &pd : nullptr, DPNN ? s_nnn : nullptr, publish);
         if constexpr (!UPDATE) publish();  // (no G2P in the P2G-only pass)
       }
       if constexpr (DPNN) {
@@ -1273,7 +1274,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
           }
           __syncthreads();
           if (pd.state == 1)
-            closed = !learned_return(d, s_mat, pd, __uint_as_float(s_nnx2[q]), par, i, &s_rec, col);
+            closed = !learned_return(d, s_mat, s_nnn, pd, __uint_as_float(s_nnx2[q]), par, i, &s_rec, col);
         }
         const unsigned nfb = (unsigned)__popcll(__ballot(closed));
         nn_fb += nfb;

@@ -316,13 +316,14 @@ __global__ __launch_bounds__(256) void k_dp_surrogate(MatDev M, NetDev N, const 
                                                       long long n, unsigned long long *used) {
   __shared__ uint2 sf[NN_TILES * 64];
   __shared__ __align__(16) float sb[NN_NBIAS];
-  nn_stage(N, sf, sb);
+  __shared__ __align__(16) float sn[NN_NORM_N];
+  nn_stage(N, sf, sb, sn);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
-  const float oscale = N.norm[NN_NORM_OUT], ooff = N.norm[NN_NORM_OUT + 1];
-  const float ylo = N.norm[NN_NORM_YLO], yhi = N.norm[NN_NORM_YHI];
+  const float oscale = sn[NN_NORM_OUT], ooff = sn[NN_NORM_OUT + 1];
+  const float ylo = sn[NN_NORM_YLO], yhi = sn[NN_NORM_YHI];
   unsigned n_net = 0, n_fb = 0;
   for (long long r0 = wave * 64; r0 < n; r0 += nwaves * 64) {
     const long long i = r0 + lane;
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(256) void k_dp_surrogate(MatDev M, NetDev N, const 
     xr[12] = epb;
     xr[13] = M.phi_deg;
     uint32_t xb[8];
-    const bool net = yield && nn_inputs(N.norm, xr, xb);
+    const bool net = yield && nn_inputs(sn, xr, xb);
     float y = 0.0f;
     const unsigned long long want = __ballot(net);
     if (want) y = nn_wave_forward<NB>(sf, sb, xb, want);  // wave-uniform

@@ -174,10 +174,13 @@ __device__ __forceinline__ float nn_wave_forward(const uint2 *sf, const float *s
   return y;
 }
 
-// Stage the fragments and biases of a network into LDS (all threads of the block).
-__device__ __forceinline__ void nn_stage(const NetDev &N, uint2 *sf, float *sb) {
+// Stage the fragments, biases and normalisation block of a network into LDS (all threads
+// of the block). The gate reads the normalisation block per sample: from LDS those are
+// broadcast reads, from global memory they were per-lane loads (64 per sample).
+__device__ __forceinline__ void nn_stage(const NetDev &N, uint2 *sf, float *sb, float *sn) {
   for (int i = threadIdx.x; i < NN_TILES * 64; i += blockDim.x) sf[i] = N.frag[i];
   for (int i = threadIdx.x; i < NN_NBIAS; i += blockDim.x) sb[i] = N.bias[i];
+  for (int i = threadIdx.x; i < NN_NORM_N; i += blockDim.x) sn[i] = N.norm[i];
 }
 
 // Normalised network inputs of a sample (raw values xr[0 .. GY_MLP_IN)) as BF16 pairs;
