@@ -317,6 +317,7 @@ int alloc_grid(gy_ctx *c) {
   if ((rc = dalloc(c, &d.offsets, (size_t)d.nkeys + 1))) return rc;
   if ((rc = dalloc(c, &d.blocksum, (size_t)d.nblocks_scan))) return rc;
   if ((rc = dalloc(c, &d.blocktiles, (size_t)d.nblocks_scan))) return rc;
+  if ((rc = dalloc(c, &d.blockbox, (size_t)d.nblocks_scan * 6))) return rc;
   if ((rc = dalloc(c, &d.active, (size_t)tile_base + 1))) return rc;
   if ((rc = dalloc(c, &d.tile_epoch, (size_t)tile_base))) return rc;
   if ((rc = dalloc(c, &d.halo_epoch, (size_t)tile_base))) return rc;
@@ -1528,7 +1529,7 @@ void gy_destroy(gy_ctx *c) {
   dfree(d.acc); dfree(d.vel); dfree(d.brick_flag); dfree(d.brick_list); dfree(d.bc_type); dfree(d.bc_val);
   dfree(d.pbc_val); dfree(d.tile_inst); dfree(d.brick_inst); dfree(d.inst); dfree(d.mats);
   dfree(d.ctrl); dfree(d.t_last); dfree(d.t_cur); dfree(d.audit);
-  dfree(d.blocktiles); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.nodemask); dfree(d.pcore); dfree(d.partial);
+  dfree(d.blocktiles); dfree(d.blockbox); dfree(d.active); dfree(d.tile_epoch); dfree(d.halo_epoch); dfree(d.nodemask); dfree(d.pcore); dfree(d.partial);
   dfree(d.acc2); dfree(d.nrm); dfree(d.claim); dfree(d.velb); dfree(d.cellobj); dfree(d.twotile);
   dfree(d.therm);
   dfree(c->nnbuf);

@@ -152,7 +152,16 @@ struct Ctrl {
   // learned return map: yielding samples through the network, and outside its trained
   // range (closed form), since the upload or the last reset
   unsigned long long nn_count[2];
+  // per epoch parity (epoch & 1 of the particle pass that writes the partials): the box of
+  // the tiles in that pass's list, i.e. the tiles it flushes, in tile coordinates, as
+  // lo[3] and TBOX_BIG - hi[3] (all zero is the whole grid: a fresh block bounds nothing).
+  // The binning's first kernel picks the entry of the pass it feeds (tbox_w) and writes
+  // per-block boxes, its second kernel writes the entry; the grid kernel reads the entry of
+  // the pass whose partials it gathers. Single-instance contexts only.
+  alignas(256) int tbox[2][6];
+  int tbox_w;
 };
+constexpr int TBOX_BIG = 1 << 30;
 
 // All device pointers of a context (passed by value to kernels).
 struct DevPtrs {
@@ -164,6 +173,8 @@ struct DevPtrs {
   int *offsets;          // per-key exclusive offsets (nkeys + 1)
   int *blocksum;         // scan partials (particle counts per scan block)
   int *blocktiles;       // scan partials (active tiles per scan block)
+  int *blockbox;         // scan partials: box of each scan block's occupied tiles (6 ints;
+                         // single-instance contexts, null where no grid kernel reads it)
   int *active;           // active tiles, longest first: entry-count classes, each in ascending tile order
   int *tile_epoch;       // per tile: epoch of its last written core partials
   int *halo_epoch;       // per tile: epoch of its last written halo-shell partials

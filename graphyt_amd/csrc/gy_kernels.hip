@@ -68,6 +68,25 @@ __global__ __launch_bounds__(WG) void k_keys(DevPtrs d) {
   if (bad) { atomicAdd(&d.ctrl->oor, 1); atomicOr(&d.ctrl->status, 1); }
 }
 
+// Block-wide minimum of six values per thread; the result in every thread. Every thread of
+// the block calls it.
+__device__ __forceinline__ void block_min6(int (&v)[6]) {
+  __shared__ int wmin[6][WG / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = min(v[k], __shfl_xor(v[k], o, 64));
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 6; k++) wmin[k][w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+#pragma unroll
+    for (int q = 0; q < WG / 64; q++) v[k] = min(v[k], wmin[k][q]);
+}
+
 // N independent block-wide exclusive scans in one pass (one shuffle chain per value, one
 // pair of barriers for all): v[k] becomes its exclusive prefix, total[k] the block total.
 template <int N>
@@ -172,6 +191,11 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
       d.ctrl->epoch += 1;
     }
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the tile box entry of the particle pass this binning feeds: it stamps epoch + 1
+    // after a step's binning, the epoch itself after an upload's
+    d.ctrl->tbox_w = (d.ctrl->epoch + advance) & 1;
+  }
   // the brick list is rebuilt by the scan's second kernel
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->active_bricks = 0;
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
@@ -191,6 +215,22 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
     d.blocksum[blockIdx.x] = t5[0];
     // tiles per bucket, at most SCAN_BLOCK / TILE3 = 64 each: one byte each
     d.blocktiles[blockIdx.x] = t5[1] | (t5[2] << 8) | (t5[3] << 16) | (t5[4] << 24);
+  }
+  if (d.ninst == 1 && d.blockbox) {  // uniform
+    // the box of this block's occupied tiles (tile coordinates: lo, TBOX_BIG - hi)
+    const InstDesc &I = d.inst[0];
+    const int tl = (int)(base / TILE3);
+    const int tc[3] = {tl % I.nt[0], (tl / I.nt[0]) % I.nt[1], tl / (I.nt[0] * I.nt[1])};
+    int v[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      v[a] = bk >= 0 ? tc[a] : TBOX_BIG;
+      v[3 + a] = bk >= 0 ? TBOX_BIG - tc[a] : TBOX_BIG;
+    }
+    block_min6(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int a = 0; a < 6; a++) d.blockbox[6 * blockIdx.x + a] = v[a];
   }
 }
 
@@ -257,6 +297,19 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
       if (b == bk) mine = preb[b] + v5[1 + b];
     }
     d.active[start + mine] = (int)(base / TILE3);
+  }
+  if (blockIdx.x == 0 && d.ninst == 1 && d.blockbox) {  // uniform
+    // the box of the listed tiles, from the first kernel's per-block boxes
+    int v[6] = {TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG};
+    for (int j = threadIdx.x; j < (int)gridDim.x; j += WG)
+#pragma unroll
+      for (int a = 0; a < 6; a++) v[a] = min(v[a], d.blockbox[6 * j + a]);
+    block_min6(v);
+    if (threadIdx.x == 0) {
+      int *bx = d.ctrl->tbox[d.ctrl->tbox_w];
+#pragma unroll
+      for (int a = 0; a < 6; a++) bx[a] = v[a];
+    }
   }
   const int allt = allb[0] + allb[1] + allb[2] + allb[3];
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allt;  // the list is in claim order
@@ -345,9 +398,20 @@ __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc
   }
 }
 
+// Whether actual node a (one axis) can hold partials of a tile in [lo, hi] on that axis:
+// its candidate tiles are (a + 1) / TILE - 1 and (a + 1) / TILE.
+__device__ __forceinline__ bool in_tile_box(int a, int lo, int hi) {
+  const int t = (a + 1) / TILE;
+  return t >= lo && t - 1 <= hi;
+}
+
 // Grid update of one flagged canonical brick (one wave per 4x4x4 brick): gather the P2G
-// partials of every actual node of the periodic class, then solve.
-__device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
+// partials of every actual node of the periodic class, then solve. box: lo[3], hi[3] of the
+// tiles that wrote partials this step; a class member outside it has none, and its gather
+// (a chain of two dependent loads per member) is skipped: the mirrors of the nodes next to
+// a periodic face usually lie at the far side of the grid, away from the particles.
+__device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch,
+                                           const int box[6]) {
   // one instance: no brick -> instance lookup in the load chain
   const InstDesc &I = d.inst[d.ninst == 1 ? 0 : d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
@@ -365,13 +429,19 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
 #pragma unroll
     for (int mz = 0; mz < 2; mz++) {
       if (mz && c2 >= 3) break;
+      const int a2 = c2 + mz * I.cper[2];
+      if (!in_tile_box(a2, box[2], box[5])) continue;
 #pragma unroll
       for (int my = 0; my < 2; my++) {
         if (my && c1 >= 3) break;
+        const int a1 = c1 + my * I.cper[1];
+        if (!in_tile_box(a1, box[1], box[4])) continue;
 #pragma unroll
         for (int mx = 0; mx < 2; mx++) {
           if (mx && c0 >= 3) break;
-          gather_partials(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], c2 + mz * I.cper[2], a7);
+          const int a0 = c0 + mx * I.cper[0];
+          if (!in_tile_box(a0, box[0], box[3])) continue;
+          gather_partials(d, I, epoch, a0, a1, a2, a7);
         }
       }
     }
@@ -434,8 +504,19 @@ __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  // the box of the tiles whose partials this step gathers (the whole grid with several
+  // instances: each has its own tile coordinates)
+  int box[6] = {0, 0, 0, TBOX_BIG, TBOX_BIG, TBOX_BIG};
+  if (d.ninst == 1) {
+    const int *bx = d.ctrl->tbox[epoch & 1];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      box[a] = bx[a];
+      box[3 + a] = TBOX_BIG - bx[3 + a];
+    }
+  }
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
-    grid_brick(d, d.brick_list[k], lane, step, epoch);
+    grid_brick(d, d.brick_list[k], lane, step, epoch, box);
 }
 
 

@@ -164,7 +164,7 @@ template <int NB = 1>
 __device__ __forceinline__ float nn_wave_forward(const uint2 *sf, const float *sb, const uint32_t x[8],
                                                  unsigned long long want) {
   static_assert(NB == 1 || NB == 2 || NB == 4, "blocks per pass");
-  constexpr unsigned long long MASK = NB == 4 ? ~0ull : ((1ull << (16 * NB)) - 1);
+  constexpr unsigned long long MASK = NB == 4 ? ~0ull : ((1ull << (16 * (NB & 3))) - 1);
   float y = 0.0f;
 #pragma unroll 1
   for (int t0 = 0; t0 < 4; t0 += NB) {

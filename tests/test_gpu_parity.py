@@ -54,6 +54,51 @@ def test_elastic_cube_500_steps(oracle_mod):
     assert audit["nonfinite"] == 0 and audit["out_of_range"] == 0
 
 
+def periodic_corner_cube(steps=200) -> W.Scene:
+    """An elastic cube in a periodic 0.5 m box (h = 0.02), translating at (2, 1.5, 1) m/s
+    while compressing (v -= 5 (x - centre)), its leading faces 1 cm short of the wrap
+    point L - 2h on all three axes: within the run particles cross the periodic corner, so
+    the nodes next to the faces gather partials from mirror nodes at both ends of the grid
+    (periodic classes of up to eight nodes, Grid.cpp:320-602)."""
+    h, L, side = 0.02, 0.5, 0.1
+    psep = h / 2
+    n = int(round(side / psep))
+    lo = L - 2 * h - 0.01 - side
+    pos = W.lattice((lo, lo, lo), (n, n, n), psep)
+    c = lo + side / 2
+    vel = np.array([2.0, 1.5, 1.0]) - 5.0 * (pos - c)
+    K, G, rho = 1e6, 5e5, 1000.0
+    s = W.Scene("periodic_corner_cube", (L, L, L), h, psep,
+                [material(model=1, bulk_mod=K, shear_mod=G, density=rho)],
+                dt=0.2 * h / np.sqrt((K + 4 * G / 3) / rho), steps=steps)
+    s.particles = W._particles(pos, vel)
+    s.node_bc_type = np.zeros(s.num_nodes, np.int32)
+    s.node_bc_val = np.zeros((s.num_nodes, 4))
+    return s
+
+
+def test_cube_crossing_the_periodic_corner(oracle_mod):
+    """3D periodic classes with partials on both sides: against the oracle re-binning every
+    step (the GPU's binning; the reference's 10-step sort cadence drops a wrapped particle's
+    contributions until its next sort)."""
+    sc, steps = periodic_corner_cube(), 200
+    g = runtime.Context(sc)
+    g.run(steps)
+    a = g.download(FIELDS)
+    audit = g.audit()
+    g.close()
+    o = oracle_mod.CpuSim(sc, "restatement", sort_every=1)
+    o.run(steps)
+    b = o.get(FIELDS)
+    o.close()
+    x0 = sc.particles.arrays["pos"]
+    wrapped = int(np.sum(np.any(b["pos"] < x0 - 0.1, axis=1)))
+    print(f"particles that crossed the wrap: {wrapped} of {sc.n}", flush=True)
+    assert wrapped > 100
+    check(a, b, {"pos": 1e-6, "vel": 1e-4, "stress": 1e-4, "density": 1e-6, "strain": 1e-4})
+    assert audit["nonfinite"] == 0 and audit["out_of_range"] == 0
+
+
 def test_dp_column_1000_steps_at_benchmark_resolution(oracle_mod):
     """Drucker-Prager column, h = 0.01, psep = 0.005, guarded apex, 1000 steps: position,
     velocity, stress and density within 1e-4 relative L2 of the FP64 oracle (reached on
