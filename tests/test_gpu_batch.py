@@ -50,6 +50,36 @@ def test_batched_equals_separate_contexts():
     b.close()
 
 
+def test_full_ensemble_batched_equals_separate_contexts():
+    """The batched-ensemble configuration at its full size (256 impacts, 4.9M particles) in
+    one batched context: every instance finite with its mass unchanged, and eight instances
+    across the size range against one-instance contexts of the same scenes (the instances
+    share no state; batching changes only the order of FP64 LDS atomics)."""
+    scenes = W.impact_ensemble()
+    steps = 100
+    b = runtime.BatchContext(scenes)
+    b.run(steps)
+    sizes = np.array([sc.n for sc in scenes])
+    print(f"{len(scenes)} instances, {int(sizes.sum())} particles", flush=True)
+    for i, sc in enumerate(scenes):
+        got = b.download(i, ["pos", "stress", "mass"])
+        assert all(np.isfinite(got[k]).all() for k in got), i
+        # m = rho * psep^3 per particle, set at the upload (Solver.cpp:38-60), never changed
+        assert np.allclose(got["mass"], sc.mats[0].density * sc.psep ** 3, rtol=1e-6, atol=0), i
+    order = np.argsort(sizes)
+    picks = sorted(set(int(order[k]) for k in np.linspace(0, len(scenes) - 1, 8).round().astype(int)))
+    for i in picks:
+        g = runtime.Context(scenes[i])
+        g.run(steps)
+        one, many = g.download(FIELDS), b.download(i, FIELDS)
+        g.close()
+        r = {k: rel(many[k], one[k]) for k in ("pos", "vel", "stress")}
+        print(f"instance {i} ({scenes[i].n} particles): batched vs alone " +
+              ", ".join(f"{k} {v:.2e}" for k, v in r.items()), flush=True)
+        assert r["pos"] <= 1e-7 and r["vel"] <= 1e-4 and r["stress"] <= 1e-4, i
+    b.close()
+
+
 def test_autotune_restores_the_particle_state():
     scenes = small_ensemble(count=5, seed=3)
     b = runtime.BatchContext(scenes)
