@@ -229,6 +229,36 @@ struct TileQueue {
   }
 };
 
+// The box (tile coordinates: box[0..2] lo, box[3..5] hi) of the tiles whose partials a
+// grid kernel gathers this step, written by the binning that fed the particle pass (Ctrl
+// tbox); the whole grid in a context of several instances (each has its own tile
+// coordinates).
+__device__ __forceinline__ void tile_box(const DevPtrs &d, int epoch, int box[6]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    box[a] = 0;
+    box[3 + a] = TBOX_BIG;
+  }
+  if (d.ninst != 1) return;
+  const int *bx = d.ctrl->tbox[epoch & 1];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    box[a] = bx[a];
+    box[3 + a] = TBOX_BIG - bx[3 + a];
+  }
+}
+
+// Whether actual node a (one axis) can hold partials of a tile in [lo, hi] on that axis,
+// for tiles of E cells per edge: its candidate tiles are (a + 1) / E - 1 and (a + 1) / E.
+// A periodic-class member outside the box has no partials, and its gather (a chain of two
+// dependent loads) is skipped: the mirrors of the nodes next to a periodic face usually lie
+// at the far side of the grid, away from the particles.
+template <int E>
+__device__ __forceinline__ bool in_tile_box(int a, int lo, int hi) {
+  const int t = (a + 1) / E;
+  return t >= lo && t - 1 <= hi;
+}
+
 // calculateColoumbFrictionContactNormals (Grid.cpp:247-308) for one body: the contact
 // normal n + min(mu, mu0) * (omega x ...) with the reference's component formulas.
 __device__ __forceinline__ void contact_normal(const float rel[3], const float n[3], float mu, float cn[3]) {

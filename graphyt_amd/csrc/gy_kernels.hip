@@ -398,18 +398,9 @@ __device__ __forceinline__ void gather_partials(const DevPtrs &d, const InstDesc
   }
 }
 
-// Whether actual node a (one axis) can hold partials of a tile in [lo, hi] on that axis:
-// its candidate tiles are (a + 1) / TILE - 1 and (a + 1) / TILE.
-__device__ __forceinline__ bool in_tile_box(int a, int lo, int hi) {
-  const int t = (a + 1) / TILE;
-  return t >= lo && t - 1 <= hi;
-}
-
 // Grid update of one flagged canonical brick (one wave per 4x4x4 brick): gather the P2G
-// partials of every actual node of the periodic class, then solve. box: lo[3], hi[3] of the
-// tiles that wrote partials this step; a class member outside it has none, and its gather
-// (a chain of two dependent loads per member) is skipped: the mirrors of the nodes next to
-// a periodic face usually lie at the far side of the grid, away from the particles.
+// partials of every actual node of the periodic class inside the tile box (tile_box),
+// then solve.
 __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch,
                                            const int box[6]) {
   // one instance: no brick -> instance lookup in the load chain
@@ -430,17 +421,17 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
     for (int mz = 0; mz < 2; mz++) {
       if (mz && c2 >= 3) break;
       const int a2 = c2 + mz * I.cper[2];
-      if (!in_tile_box(a2, box[2], box[5])) continue;
+      if (!in_tile_box<TILE>(a2, box[2], box[5])) continue;
 #pragma unroll
       for (int my = 0; my < 2; my++) {
         if (my && c1 >= 3) break;
         const int a1 = c1 + my * I.cper[1];
-        if (!in_tile_box(a1, box[1], box[4])) continue;
+        if (!in_tile_box<TILE>(a1, box[1], box[4])) continue;
 #pragma unroll
         for (int mx = 0; mx < 2; mx++) {
           if (mx && c0 >= 3) break;
           const int a0 = c0 + mx * I.cper[0];
-          if (!in_tile_box(a0, box[0], box[3])) continue;
+          if (!in_tile_box<TILE>(a0, box[0], box[3])) continue;
           gather_partials(d, I, epoch, a0, a1, a2, a7);
         }
       }
@@ -504,17 +495,8 @@ __global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
-  // the box of the tiles whose partials this step gathers (the whole grid with several
-  // instances: each has its own tile coordinates)
-  int box[6] = {0, 0, 0, TBOX_BIG, TBOX_BIG, TBOX_BIG};
-  if (d.ninst == 1) {
-    const int *bx = d.ctrl->tbox[epoch & 1];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      box[a] = bx[a];
-      box[3 + a] = TBOX_BIG - bx[3 + a];
-    }
-  }
+  int box[6];
+  tile_box(d, epoch, box);
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
     grid_brick(d, d.brick_list[k], lane, step, epoch, box);
 }
@@ -1716,7 +1698,8 @@ __global__ __launch_bounds__(WG) void k_mb_p2g(DevPtrs d) {
 // claims and normals (Grid.cpp:9-245), BCs on the centre of mass and both bodies
 // (Grid.cpp:967-1031). VEL gets the centre-of-mass velocity p/m, its acceleration and m;
 // VELB the body velocities and accelerations G2P reads.
-__device__ __forceinline__ void mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
+__device__ __forceinline__ void mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch,
+                                              const int box[6]) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = (lb / I.nb[0]) % I.nb[1], bz = lb / (I.nb[0] * I.nb[1]);
@@ -1727,11 +1710,17 @@ __device__ __forceinline__ void mb_grid_brick(const DevPtrs &d, int gb, int lane
     float a7[7] = {0, 0, 0, 0, 0, 0, 0}, ab[MB_ACC2_W];
     for (int mz = 0; mz < 2; mz++) {
       if (mz && c2 >= 3) break;
+      const int a2 = c2 + mz * I.cper[2];
+      if (!in_tile_box<TILE>(a2, box[2], box[5])) continue;
       for (int my = 0; my < 2; my++) {
         if (my && c1 >= 3) break;
+        const int a1 = c1 + my * I.cper[1];
+        if (!in_tile_box<TILE>(a1, box[1], box[4])) continue;
         for (int mx = 0; mx < 2; mx++) {
           if (mx && c0 >= 3) break;
-          gather_partials(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], c2 + mz * I.cper[2], a7);
+          const int a0 = c0 + mx * I.cper[0];
+          if (!in_tile_box<TILE>(a0, box[0], box[3])) continue;
+          gather_partials(d, I, epoch, a0, a1, a2, a7);
         }
       }
     }
@@ -1768,8 +1757,10 @@ __global__ __launch_bounds__(WG) void k_mb_grid(DevPtrs d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  int box[6];
+  tile_box(d, epoch, box);
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
-    mb_grid_brick(d, d.brick_list[k], lane, step, epoch);
+    mb_grid_brick(d, d.brick_list[k], lane, step, epoch, box);
 }
 
 // FP64 audit: per block sums of mass, momentum, kinetic energy, a stress-energy proxy and

@@ -105,7 +105,8 @@ __device__ __forceinline__ void plane_gather(const DevPtrs &d, const InstDesc &I
 // Nodal solve of one flagged plane brick (one wave, lane = node of the 8 x 8 brick): the
 // P2G sums of the periodic class, v = p/m, a = f/m + g, the kick (half at step 1), BCs per
 // actual node (Grid.cpp:9-245, 862-1031), fan-out to every node of the class.
-__device__ __forceinline__ void plane_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
+__device__ __forceinline__ void plane_brick(const DevPtrs &d, int gb, int lane, int step, int epoch,
+                                            const int box[6]) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int bx = lb % I.nb[0], by = lb / I.nb[0];
@@ -115,10 +116,14 @@ __device__ __forceinline__ void plane_brick(const DevPtrs &d, int gb, int lane, 
 #pragma unroll
     for (int my = 0; my < 2; my++) {
       if (my && c1 >= 3) break;
+      const int a1 = c1 + my * I.cper[1];
+      if (!in_tile_box<PT>(a1, box[1], box[4])) continue;
 #pragma unroll
       for (int mx = 0; mx < 2; mx++) {
         if (mx && c0 >= 3) break;
-        plane_gather(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], a7);
+        const int a0 = c0 + mx * I.cper[0];
+        if (!in_tile_box<PT>(a0, box[0], box[3])) continue;
+        plane_gather(d, I, epoch, a0, a1, a7);
       }
     }
     // particles scattered straight to the grid (outside their tile's region)
@@ -172,8 +177,10 @@ __global__ __launch_bounds__(WG) void k_plane_grid(DevPtrs d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  int box[6];
+  tile_box(d, epoch, box);  // plane tiles: x, y tile coordinates, one layer
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
-    plane_brick(d, d.brick_list[k], lane, step, epoch);
+    plane_brick(d, d.brick_list[k], lane, step, epoch, box);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -804,7 +811,8 @@ __global__ __launch_bounds__(WG) void k_plane_mb_p2g(DevPtrs d) {
 // Contact nodal solve on the flagged plane bricks (one wave per 8 x 8 brick): centre-of-mass
 // sums of the periodic class (tile partials and direct scatter), body sums, then per actual
 // node its own claims, normals, solve and BCs (mb_solve_node).
-__device__ __forceinline__ void plane_mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch) {
+__device__ __forceinline__ void plane_mb_grid_brick(const DevPtrs &d, int gb, int lane, int step, int epoch,
+                                                    const int box[6]) {
   const InstDesc &I = d.inst[d.brick_inst[gb]];
   const int lb = gb - I.brick_base;
   const int c0 = (lb % I.nb[0]) * PB + (lane & 7), c1 = (lb / I.nb[0]) * PB + (lane >> 3);
@@ -813,9 +821,13 @@ __device__ __forceinline__ void plane_mb_grid_brick(const DevPtrs &d, int gb, in
     float a7[7] = {0, 0, 0, 0, 0, 0, 0}, ab[MB_ACC2_W];
     for (int my = 0; my < 2; my++) {
       if (my && c1 >= 3) break;
+      const int a1 = c1 + my * I.cper[1];
+      if (!in_tile_box<PT>(a1, box[1], box[4])) continue;
       for (int mx = 0; mx < 2; mx++) {
         if (mx && c0 >= 3) break;
-        plane_gather(d, I, epoch, c0 + mx * I.cper[0], c1 + my * I.cper[1], a7);
+        const int a0 = c0 + mx * I.cper[0];
+        if (!in_tile_box<PT>(a0, box[0], box[3])) continue;
+        plane_gather(d, I, epoch, a0, a1, a7);
       }
     }
     take_node_sums(d, ci, a7, ab);
@@ -845,8 +857,10 @@ __global__ __launch_bounds__(WG) void k_plane_mb_grid(DevPtrs d) {
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
+  int box[6];
+  tile_box(d, epoch, box);
   for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
-    plane_mb_grid_brick(d, d.brick_list[k], lane, step, epoch);
+    plane_mb_grid_brick(d, d.brick_list[k], lane, step, epoch, box);
 }
 
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
