@@ -108,7 +108,7 @@ def cpu_baseline(sample_steps: int, warm: int, workload: str = "dp_column") -> d
 
 
 def run_cpu_baseline(cpu_steps: int, workload: str = "dp_column"):
-    """The CPU leg in a child process with the threads pinned as BASELINE.md asks
+    """The CPU leg in a child process with the threads pinned to the host's cores
     (OMP_PROC_BIND=close, OMP_PLACES=cores unless set)."""
     env = dict(os.environ)
     env.setdefault("OMP_PROC_BIND", "close")
@@ -442,8 +442,8 @@ def main():
         print(json.dumps(cpu_baseline(args.cpu_steps, 2, args.workload)))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs the MPM step on one GPU; multi-GPU runs are not "
-              "part of it (see DESIGN.md for how the step would shard)", file=sys.stderr)
+        print("bench.py: the MPM step runs on one GPU; multi-GPU runs (slabs along x with a "
+              "3-plane overlap exchange, or instances spread over GPUs) are not built", file=sys.stderr)
         return 2
 
     from graphyt_amd import runtime, workloads as W

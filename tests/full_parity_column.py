@@ -3,7 +3,7 @@ guarded apex): the GPU step against the FP64 oracle restatement on the host's co
 `steps` steps, relative L2 over all particles per field. Not collected by pytest (the
 oracle takes ~0.4 s per step at this size); run as
 
-    python3 tests/full_parity_column.py [steps]
+    python3 tests/full_parity_column.py [steps [result.json]]
 """
 import json
 import os
@@ -26,7 +26,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def main(steps: int = 1000) -> int:
+def main(steps: int = 1000, out_path: str = "") -> int:
     sc = W.dp_column()
     t0 = time.time()
     g = runtime.Context(sc)
@@ -43,11 +43,12 @@ def main(steps: int = 1000) -> int:
     out.update(steps=steps, particles=sc.n, max_abs_pos_diff=float(np.abs(a["pos"] - b["pos"]).max()),
                nonfinite_gpu=int(np.isnan(a["stress"]).sum()), nonfinite_oracle=int(np.isnan(b["stress"]).sum()))
     print(json.dumps(out), flush=True)
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", f"full_parity_{steps}.json"), "w") as f:
-        json.dump(out, f)
+    if out_path:
+        os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
+        with open(out_path, "w") as f:
+            json.dump(out, f)
     return 0
 
 
 if __name__ == "__main__":
-    sys.exit(main(int(sys.argv[1]) if len(sys.argv) > 1 else 1000))
+    sys.exit(main(int(sys.argv[1]) if len(sys.argv) > 1 else 1000, sys.argv[2] if len(sys.argv) > 2 else ""))
