@@ -90,6 +90,30 @@ def test_contact_graph_replay_matches_eager():
         assert rel(outs[1][k], outs[0][k]) <= tol, k
 
 
+def test_brazilian_disk_damage_growth(oracle_mod):
+    """The same disk with the platens at 20 m/s for 600 steps: the contact load takes the
+    strain past the Grady-Kipp activation strain and damage grows in ~2400 of the 19,960
+    particles (FP64 eig3 of every stressed particle, DamageModels.cpp). Reached on MI355X:
+    the same 2376 particles damaged as in the oracle, damage 1.2e-6, stress 4.3e-7, s_max
+    9.1e-7, velocity 6.3e-7 relative L2."""
+    sc = W.gk_disk(h=0.004, v=20.0, steps=600)
+    g = runtime.Context(sc)
+    g.run(600)
+    a = g.download(["pos", "vel", "stress", "damage", "smax"])
+    g.close()
+    o = oracle_mod.CpuSim(sc, "restatement")
+    o.run(600)
+    b = o.get(["pos", "vel", "stress", "damage", "smax"])
+    grown = int((b["damage"] > 0).sum())
+    print(f"damaged particles: oracle {grown}, GPU {int((a['damage'] > 0).sum())}; "
+          f"max damage {b['damage'].max():.3f}", flush=True)
+    assert grown > 1000
+    out = {k: rel(a[k], b[k]) for k in ("pos", "vel", "stress", "damage", "smax")}
+    print(", ".join(f"{k} {v:.2e}" for k, v in out.items()), flush=True)
+    for k, tol in (("pos", 1e-6), ("vel", 1e-4), ("stress", 1e-4), ("damage", 1e-4), ("smax", 1e-4)):
+        assert out[k] <= tol, k
+
+
 def test_brazilian_disk_with_rigid_platens(oracle_mod):
     """The Grady-Kipp disk between two rigid platens (three objects), scaled to h = 0.004
     (19,960 particles), 300 steps: contact, rigid override and the FP64 eig3 path.
