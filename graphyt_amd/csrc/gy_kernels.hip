@@ -162,13 +162,14 @@ __device__ __forceinline__ int tile_bucket(int n) {
 // With `advance`, block 0 also performs the step bookkeeping: times, step, parity and
 // epoch (no block of this kernel reads them; the kernels after it see the new values).
 // Append the flagged bricks among gb, gb + stride, ... to the brick list: one atomic per
-// wave and round. All lanes of the wave take part (uniform trip count).
-__device__ __forceinline__ void append_bricks(const DevPtrs &d, int gb, int stride) {
+// wave and round. All lanes of the wave take part (uniform trip count). first >= 0: the
+// raw flag byte of brick min(gb, nbricks - 1), loaded by the caller.
+__device__ __forceinline__ void append_bricks(const DevPtrs &d, int gb, int stride, int first = -1) {
   const int lane = threadIdx.x & 63;
   const int wave0 = gb - lane;
   for (int w = wave0; w < d.nbricks; w += stride) {
     const int b = w + lane;
-    const bool f = b < d.nbricks && d.brick_flag[b] != 0;
+    const bool f = b < d.nbricks && ((w == wave0 && first >= 0) ? first != 0 : d.brick_flag[b] != 0);
     const unsigned long long m = __ballot(f);
     if (m == 0ull) continue;
     const int leader = __ffsll((long long)m) - 1;
@@ -235,6 +236,22 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
 }
 
 __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
+  // The loads that depend on nothing computed here go out first, so that their round trips
+  // overlap: this thread's counts, its first brick flag and (block 0) the per-block tile
+  // boxes; then the prefixes of the earlier blocks.
+  const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+  const bool have = base < d.nkeys;
+  int c[SCAN_ITEMS];
+  if (have) load16(d.count + base, c);
+  const int gb = blockIdx.x * WG + threadIdx.x;
+  int flag0 = 0;  // compared where used; contexts without a grid (the sorts) have no bricks
+  if (d.nbricks > 0) flag0 = d.brick_flag[min(gb, d.nbricks - 1)];
+  const bool boxes = blockIdx.x == 0 && d.ninst == 1 && d.blockbox;  // uniform
+  int box[6] = {TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG};
+  if (boxes)
+    for (int j = threadIdx.x; j < (int)gridDim.x; j += WG)
+#pragma unroll
+      for (int a = 0; a < 6; a++) box[a] = min(box[a], d.blockbox[6 * j + a]);
   // prefixes of all earlier blocks (particles and active tiles per bucket), and the totals
   int pre = 0, preb[NBUCKET] = {0, 0, 0, 0}, allb[NBUCKET] = {0, 0, 0, 0};
   for (int j = threadIdx.x; j < (int)gridDim.x; j += WG) {
@@ -254,11 +271,7 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
 #pragma unroll
     for (int b = 0; b < NBUCKET; b++) { preb[b] = t9[1 + b]; allb[b] = t9[5 + b]; }
   }
-  const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
-  const bool have = base < d.nkeys;
-  int c[SCAN_ITEMS];
   int s = 0;
-  if (have) load16(d.count + base, c);
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; j++) {
     if (!have) c[j] = 0;
@@ -298,17 +311,13 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     }
     d.active[start + mine] = (int)(base / TILE3);
   }
-  if (blockIdx.x == 0 && d.ninst == 1 && d.blockbox) {  // uniform
+  if (boxes) {
     // the box of the listed tiles, from the first kernel's per-block boxes
-    int v[6] = {TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG};
-    for (int j = threadIdx.x; j < (int)gridDim.x; j += WG)
-#pragma unroll
-      for (int a = 0; a < 6; a++) v[a] = min(v[a], d.blockbox[6 * j + a]);
-    block_min6(v);
+    block_min6(box);
     if (threadIdx.x == 0) {
       int *bx = d.ctrl->tbox[d.ctrl->tbox_w];
 #pragma unroll
-      for (int a = 0; a < 6; a++) bx[a] = v[a];
+      for (int a = 0; a < 6; a++) bx[a] = box[a];
     }
   }
   const int allt = allb[0] + allb[1] + allb[2] + allb[3];
@@ -319,7 +328,7 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   // list of the bricks the particle kernel flagged, for the next grid update
-  append_bricks(d, blockIdx.x * WG + threadIdx.x, gridDim.x * WG);
+  append_bricks(d, gb, gridDim.x * WG, flag0);
 }
 
 // Brick list outside the step sequence (after the first P2G of an upload).
