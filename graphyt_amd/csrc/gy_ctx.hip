@@ -30,7 +30,7 @@ uint32_t bf16_bits_rn(float x) {
 
 const char *kKernelNames[K_COUNT_] = {"keys",  "scan_partial", "scan_final", "place",   "p2g",
                                       "grid",  "g2p2g",        "advance",    "audit",   "order",
-                                      "mb_claim", "mb_p2g", "therm_p2g", "therm_g2p"};
+                                      "mb_claim", "mb_p2g", "therm_p2g", "therm_g2p", "scan_final_grid"};
 
 struct Timed {
   int id;
@@ -67,7 +67,9 @@ struct gy_ctx {
   // options
   bool use_graph = true;
   bool profile = false;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;       // one step (STEP_FULL)
+  hipGraphExec_t graph_ahead = nullptr; // one step with the next step's grid update (STEP_AHEAD)
+  hipGraphExec_t graph_last = nullptr;  // one step after a STEP_AHEAD (STEP_LAST)
   bool graph_valid = false;
   std::vector<Timed> pending;
   int64_t launches[K_COUNT_] = {0};
@@ -358,8 +360,19 @@ int check_status(gy_ctx *c) {
   return GY_OK;
 }
 
+// How a step is enqueued. STEP_FULL: grid update, particle kernel, binning. Within a call
+// of several steps of a single-body 3D context without heat transfer, the grid update of
+// the next step runs beside the second scan kernel (they share no data): the call starts
+// with the first step's grid update, then come STEP_AHEAD steps (particle kernel, binning
+// with the next step's grid update) and one STEP_LAST step (particle kernel, binning), so
+// that between calls the state is the same as after STEP_FULL steps. Every mode runs the
+// same kernels on the same inputs.
+enum StepMode { STEP_FULL = 0, STEP_AHEAD, STEP_LAST };
+
+bool grid_ahead_ok(const gy_ctx *c) { return !c->plane && !c->d.multibody && !c->d.thermal; }
+
 // One step after the binning of the current buffer is in place.
-void enqueue_step(gy_ctx *c) {
+void enqueue_step(gy_ctx *c, StepMode mode = STEP_FULL) {
   void *s = c->stream;
   const bool th = c->d.thermal != 0;
   if (c->plane && c->d.multibody) {
@@ -394,6 +407,17 @@ void enqueue_step(gy_ctx *c) {
     timed(c, K_MB_CLAIM, [&] { launch_mb_claim(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_mb_p2g(c->d, s); });
     if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
+    return;
+  }
+  if (mode != STEP_FULL && grid_ahead_ok(c)) {
+    // the grid update of this step has run (in the previous step's second scan kernel, or
+    // on its own at the start of the call)
+    const bool ahead = mode == STEP_AHEAD;
+    timed(c, K_G2P2G, [&] { launch_g2p2g(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1, ahead ? 1 : 0); });
+    if (ahead) timed(c, K_SCAN_GRID, [&] { launch_scan_final_grid(c->d, s); });
+    else timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
     return;
   }
   timed(c, K_GRID, [&] { launch_grid(c->d, s); });
@@ -444,23 +468,33 @@ void enqueue_init(gy_ctx *c) {
 }
 
 void drop_graph(gy_ctx *c) {
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  c->graph = nullptr;
+  for (hipGraphExec_t *g : {&c->graph, &c->graph_ahead, &c->graph_last}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
+  }
   c->graph_valid = false;
 }
 
-int build_graph(gy_ctx *c) {
-  drop_graph(c);
+int capture_step(gy_ctx *c, StepMode mode, hipGraphExec_t *out) {
   hipGraph_t g;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const bool prof = c->profile;
   c->profile = false;
-  enqueue_step(c);
+  enqueue_step(c, mode);
   c->profile = prof;
   HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
-  hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(c, GY_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  return GY_OK;
+}
+
+int build_graph(gy_ctx *c) {
+  drop_graph(c);
+  int rc = capture_step(c, STEP_FULL, &c->graph);
+  if (!rc && grid_ahead_ok(c)) rc = capture_step(c, STEP_AHEAD, &c->graph_ahead);
+  if (!rc && grid_ahead_ok(c)) rc = capture_step(c, STEP_LAST, &c->graph_last);
+  if (rc) return rc;
   c->graph_valid = true;
   return GY_OK;
 }
@@ -751,11 +785,20 @@ static int iterate_impl(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
     enqueue_init(c);
     c->acc_valid = true;
   }
-  if (c->use_graph && !c->profile && nsteps > 1) {
-    if (!c->graph_valid) {
-      int rc = build_graph(c);
-      if (rc) return rc;
+  const bool graph = c->use_graph && !c->profile && nsteps > 1;
+  if (graph && !c->graph_valid) {
+    int rc = build_graph(c);
+    if (rc) return rc;
+  }
+  if (nsteps > 1 && grid_ahead_ok(c)) {
+    // this step's grid update, then each step's binning beside the next step's grid update
+    timed(c, K_GRID, [&] { launch_grid(c->d, c->stream); });
+    for (int i = 0; i < nsteps; i++) {
+      const StepMode m = i + 1 < nsteps ? STEP_AHEAD : STEP_LAST;
+      if (graph) HIPCHECK(c, hipGraphLaunch(m == STEP_AHEAD ? c->graph_ahead : c->graph_last, c->stream));
+      else enqueue_step(c, m);
     }
+  } else if (graph) {
     for (int i = 0; i < nsteps; i++) HIPCHECK(c, hipGraphLaunch(c->graph, c->stream));
   } else {
     for (int i = 0; i < nsteps; i++) enqueue_step(c);

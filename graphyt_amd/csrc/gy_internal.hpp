@@ -230,7 +230,7 @@ constexpr int MB_VELB_W = 12;   // per node: v1 a1 v2 a2
 // Kernel identifiers for per-kernel timing.
 enum KernelId : int {
   K_KEYS = 0, K_SCAN_PARTIAL, K_SCAN_FINAL, K_PLACE, K_P2G, K_GRID, K_G2P2G, K_ADVANCE,
-  K_AUDIT, K_ORDER, K_MB_CLAIM, K_MB_P2G, K_THERM_P2G, K_THERM_G2P, K_COUNT_
+  K_AUDIT, K_ORDER, K_MB_CLAIM, K_MB_P2G, K_THERM_P2G, K_THERM_G2P, K_SCAN_GRID, K_COUNT_
 };
 
 // Device material constants of a gy_material (defined in gy_ctx.hip).
@@ -246,8 +246,11 @@ int mlp_export(gy_mlp *m, uint2 *frag, float *bias, float *norm);
 void launch_set_ctrl(const DevPtrs &d, int step, double t, int reset, void *stream);
 void launch_keys(const DevPtrs &d, void *stream);
 void launch_hist(const DevPtrs &d, void *stream);  // histogram of key[0] (standalone sort)
-void launch_scan_partial(const DevPtrs &d, void *stream, int advance = 0);  // advance: + step bookkeeping
+// advance: + step bookkeeping; bricks: + the flagged-brick list (for launch_scan_final_grid)
+void launch_scan_partial(const DevPtrs &d, void *stream, int advance = 0, int bricks = 0);
 void launch_scan_final(const DevPtrs &d, void *stream);
+// the second scan kernel beside the next step's grid update (one launch)
+void launch_scan_final_grid(const DevPtrs &d, void *stream);
 void launch_place(const DevPtrs &d, void *stream);
 void launch_p2g_only(const DevPtrs &d, void *stream);
 void launch_grid(const DevPtrs &d, void *stream);

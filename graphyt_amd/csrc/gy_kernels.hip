@@ -180,7 +180,9 @@ __device__ __forceinline__ void append_bricks(const DevPtrs &d, int gb, int stri
   }
 }
 
-__global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
+// bricks: this kernel also lists the flagged bricks (the grid update then runs beside the
+// second scan kernel); the list's length was reset by the particle kernel before it.
+__global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance, int bricks) {
   if (advance && blockIdx.x == 0) {
     for (int i = threadIdx.x; i < d.ninst; i += WG) {
       d.t_last[i] = d.t_cur[i];
@@ -197,8 +199,8 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
     // after a step's binning, the epoch itself after an upload's
     d.ctrl->tbox_w = (d.ctrl->epoch + advance) & 1;
   }
-  // the brick list is rebuilt by the scan's second kernel
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->active_bricks = 0;
+  // the brick list is rebuilt by the scan's second kernel (or below)
+  if (!bricks && blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->active_bricks = 0;
   const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   if (base < d.nkeys) {
@@ -233,36 +235,38 @@ __global__ __launch_bounds__(WG) void k_scan_partial(DevPtrs d, int advance) {
 #pragma unroll
       for (int a = 0; a < 6; a++) d.blockbox[6 * blockIdx.x + a] = v[a];
   }
+  if (bricks) append_bricks(d, blockIdx.x * WG + threadIdx.x, gridDim.x * WG);
 }
 
-__global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
+// The second scan kernel as block bid of nblk; bricks: it also lists the flagged bricks.
+__device__ __forceinline__ void scan_final_body(const DevPtrs &d, int bid, int nblk, bool bricks) {
   // The loads that depend on nothing computed here go out first, so that their round trips
   // overlap: this thread's counts, its first brick flag and (block 0) the per-block tile
   // boxes; then the prefixes of the earlier blocks.
-  const long long base = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+  const long long base = (long long)bid * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
   const bool have = base < d.nkeys;
   int c[SCAN_ITEMS];
   if (have) load16(d.count + base, c);
-  const int gb = blockIdx.x * WG + threadIdx.x;
+  const int gb = bid * WG + threadIdx.x;
   int flag0 = 0;  // compared where used; contexts without a grid (the sorts) have no bricks
-  if (d.nbricks > 0) flag0 = d.brick_flag[min(gb, d.nbricks - 1)];
-  const bool boxes = blockIdx.x == 0 && d.ninst == 1 && d.blockbox;  // uniform
+  if (bricks && d.nbricks > 0) flag0 = d.brick_flag[min(gb, d.nbricks - 1)];
+  const bool boxes = bid == 0 && d.ninst == 1 && d.blockbox;  // uniform
   int box[6] = {TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG, TBOX_BIG};
   if (boxes)
-    for (int j = threadIdx.x; j < (int)gridDim.x; j += WG)
+    for (int j = threadIdx.x; j < nblk; j += WG)
 #pragma unroll
       for (int a = 0; a < 6; a++) box[a] = min(box[a], d.blockbox[6 * j + a]);
   // prefixes of all earlier blocks (particles and active tiles per bucket), and the totals
   int pre = 0, preb[NBUCKET] = {0, 0, 0, 0}, allb[NBUCKET] = {0, 0, 0, 0};
-  for (int j = threadIdx.x; j < (int)gridDim.x; j += WG) {
+  for (int j = threadIdx.x; j < nblk; j += WG) {
     const int bt = d.blocktiles[j];
 #pragma unroll
     for (int b = 0; b < NBUCKET; b++) {
       const int nb = (bt >> (8 * b)) & 0xff;
-      if (j < (int)blockIdx.x) preb[b] += nb;
+      if (j < bid) preb[b] += nb;
       allb[b] += nb;
     }
-    if (j < (int)blockIdx.x) pre += d.blocksum[j];
+    if (j < bid) pre += d.blocksum[j];
   }
   {
     int v9[9] = {pre, preb[0], preb[1], preb[2], preb[3], allb[0], allb[1], allb[2], allb[3]}, t9[9];
@@ -321,15 +325,17 @@ __global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) {
     }
   }
   const int allt = allb[0] + allb[1] + allb[2] + allb[3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allt;  // the list is in claim order
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) d.ctrl->heavy_tiles = allt;  // the list is in claim order
+  if (bid == nblk - 1 && threadIdx.x == 0) {
     d.offsets[d.nkeys] = pre + tot;
     d.ctrl->active_tiles = allt;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  if (bid == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   // list of the bricks the particle kernel flagged, for the next grid update
-  append_bricks(d, gb, gridDim.x * WG, flag0);
+  if (bricks) append_bricks(d, gb, nblk * WG, flag0);
 }
+
+__global__ __launch_bounds__(WG) void k_scan_final(DevPtrs d) { scan_final_body(d, blockIdx.x, gridDim.x, true); }
 
 // Brick list outside the step sequence (after the first P2G of an upload).
 __global__ void k_brick_list_reset(DevPtrs d) { d.ctrl->active_bricks = 0; }
@@ -499,15 +505,29 @@ __device__ __forceinline__ void grid_brick(const DevPtrs &d, int gb, int lane, i
 }
 
 // Grid update, persistent over the list of flagged bricks.
-__global__ __launch_bounds__(WG) void k_grid(DevPtrs d) {
+// The grid update as block bid of nblk: one wave per listed brick.
+__device__ __forceinline__ void grid_body(const DevPtrs &d, int bid, int nblk) {
   // the particle kernel that follows walks the active tile list from the start
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  if (bid == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
   int box[6];
   tile_box(d, epoch, box);
-  for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
+  for (int k = bid * (WG / 64) + wave; k < n; k += nblk * (WG / 64))
     grid_brick(d, d.brick_list[k], lane, step, epoch, box);
+}
+
+__global__ __launch_bounds__(WG) void k_grid(DevPtrs d) { grid_body(d, blockIdx.x, gridDim.x); }
+
+// The second scan kernel of step s and the grid update of step s + 1 in one launch: blocks
+// [0, ngrid) update the grid, the others scan. The two share no data: the grid update
+// reads the tile partials of step s, the brick list (listed by the first scan kernel) and
+// the tile box of the pass that wrote those partials; the scan writes the offsets, the
+// active tile list and the other tile box entry. Both are chains of dependent round trips
+// with little work, so the shorter one hides beside the other.
+__global__ __launch_bounds__(WG) void k_scan_final_grid(DevPtrs d, int ngrid) {
+  if ((int)blockIdx.x < ngrid) grid_body(d, blockIdx.x, ngrid);
+  else scan_final_body(d, blockIdx.x - ngrid, gridDim.x - ngrid, false);
 }
 
 
@@ -1148,6 +1168,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(MB ? 2 : WAV
   const int nh = cld(&d.ctrl->heavy_tiles), na = cld(&d.ctrl->active_tiles);
   unsigned nn_net = 0, nn_fb = 0;  // this wave's network and fallback samples
   if constexpr (DPNN) nn_stage(d.nn, s_nnw, s_nnb, s_nnn);  // visible after the loop's first barrier
+  // the grid update before this kernel has read the brick list: the bricks this kernel
+  // flags are listed from an empty list (by either scan kernel)
+  if (blockIdx.x == 0 && tid == 0) d.ctrl->active_bricks = 0;
 
   // A tile's successor is claimed when its last chunk starts (one work counter, longest
   // tiles first); the successor's offsets and node velocities are loaded during that
@@ -1844,8 +1867,8 @@ void launch_keys(const DevPtrs &d, void *st) {
 void launch_hist(const DevPtrs &d, void *st) {
   if (d.np > 0) hipLaunchKernelGGL(k_hist, dim3(blocks(d.np, WG)), dim3(WG), 0, S(st), d);
 }
-void launch_scan_partial(const DevPtrs &d, void *st, int advance) {
-  hipLaunchKernelGGL(k_scan_partial, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d, advance);
+void launch_scan_partial(const DevPtrs &d, void *st, int advance, int bricks) {
+  hipLaunchKernelGGL(k_scan_partial, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d, advance, bricks);
 }
 void launch_scan_final(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_scan_final, dim3(d.nblocks_scan), dim3(WG), 0, S(st), d);
@@ -1883,10 +1906,16 @@ void launch_fused(const DevPtrs &d, hipStream_t st) {
 }
 
 void launch_p2g_only(const DevPtrs &d, void *st) { launch_fused<false>(d, S(st)); }
+// persistent: grid_factor workgroups per persistent particle workgroup at most
+static int grid_blocks(const DevPtrs &d) {
+  return std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
+}
 void launch_grid(const DevPtrs &d, void *st) {
-  // persistent: grid_factor workgroups per persistent particle workgroup at most
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
-  hipLaunchKernelGGL(k_grid, dim3(nb), dim3(WG), 0, S(st), d);
+  hipLaunchKernelGGL(k_grid, dim3(grid_blocks(d)), dim3(WG), 0, S(st), d);
+}
+void launch_scan_final_grid(const DevPtrs &d, void *st) {
+  const int ng = grid_blocks(d);
+  hipLaunchKernelGGL(k_scan_final_grid, dim3(ng + d.nblocks_scan), dim3(WG), 0, S(st), d, ng);
 }
 void launch_brick_list(const DevPtrs &d, void *st) {
   hipLaunchKernelGGL(k_brick_list_reset, dim3(1), dim3(1), 0, S(st), d);

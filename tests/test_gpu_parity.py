@@ -243,6 +243,27 @@ def test_graph_replay_matches_eager_launches():
     check(outs[1], outs[0], {"pos": 1e-7, "vel": 1e-5, "stress": 1e-4})
 
 
+def test_grid_update_beside_the_scan_matches_single_steps():
+    """A call of several steps runs each later step's grid update in the same launch as the
+    step before's second scan kernel (graph replay and eager launches); a call of one step
+    runs it on its own. 60 one-step calls against one 60-step call, in both launch modes,
+    agree to rounding (the FP64 LDS sums are order-dependent)."""
+    sc = W.dp_column_small()
+    for graph in (True, False):
+        outs = []
+        for single in (True, False):
+            g = runtime.Context(sc, graph=graph)
+            if single:
+                for s in range(60):
+                    g.run(1, step0=s)
+            else:
+                g.run(60)
+            outs.append(g.download(["pos", "vel", "stress", "density"]))
+            assert g.audit()["nonfinite"] == 0
+            g.close()
+        check(outs[1], outs[0], {"pos": 1e-7, "vel": 1e-5, "stress": 1e-4, "density": 1e-6})
+
+
 def test_repeat_runs_agree():
     """FP32 atomics make summation order run-dependent; results agree to rounding."""
     sc = W.dp_column_small()
