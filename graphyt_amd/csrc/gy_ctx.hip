@@ -361,7 +361,7 @@ int check_status(gy_ctx *c) {
 }
 
 // How a step is enqueued. STEP_FULL: grid update, particle kernel, binning. Within a call
-// of several steps of a single-body 3D context without heat transfer, the grid update of
+// of several steps of a single-body context (3D or plane) without heat transfer, the grid update of
 // the next step runs beside the second scan kernel (they share no data): the call starts
 // with the first step's grid update, then come STEP_AHEAD steps (particle kernel, binning
 // with the next step's grid update) and one STEP_LAST step (particle kernel, binning), so
@@ -369,7 +369,7 @@ int check_status(gy_ctx *c) {
 // same kernels on the same inputs.
 enum StepMode { STEP_FULL = 0, STEP_AHEAD, STEP_LAST };
 
-bool grid_ahead_ok(const gy_ctx *c) { return !c->plane && !c->d.multibody && !c->d.thermal; }
+bool grid_ahead_ok(const gy_ctx *c) { return !c->d.multibody && !c->d.thermal; }
 
 // One step after the binning of the current buffer is in place.
 void enqueue_step(gy_ctx *c, StepMode mode = STEP_FULL) {
@@ -385,6 +385,15 @@ void enqueue_step(gy_ctx *c, StepMode mode = STEP_FULL) {
     timed(c, K_MB_CLAIM, [&] { launch_mb_cells(c->d, s); launch_plane_mb_claim(c->d, s); });
     timed(c, K_MB_P2G, [&] { launch_plane_mb_p2g(c->d, s); });
     if (th) timed(c, K_THERM_P2G, [&] { launch_therm_p2g(c->d, s); });
+    return;
+  }
+  if (c->plane && mode != STEP_FULL && grid_ahead_ok(c)) {  // as the 3D path below
+    const bool ahead = mode == STEP_AHEAD;
+    timed(c, K_G2P2G, [&] { launch_plane_g2p2g(c->d, s); });
+    timed(c, K_SCAN_PARTIAL, [&] { launch_scan_partial(c->d, s, 1, ahead ? 1 : 0); });
+    if (ahead) timed(c, K_SCAN_GRID, [&] { launch_plane_scan_final_grid(c->d, s); });
+    else timed(c, K_SCAN_FINAL, [&] { launch_scan_final(c->d, s); });
+    timed(c, K_PLACE, [&] { launch_place(c->d, s); });
     return;
   }
   if (c->plane) {
@@ -792,7 +801,10 @@ static int iterate_impl(gy_ctx *c, double t0, int32_t step0, int32_t nsteps) {
   }
   if (nsteps > 1 && grid_ahead_ok(c)) {
     // this step's grid update, then each step's binning beside the next step's grid update
-    timed(c, K_GRID, [&] { launch_grid(c->d, c->stream); });
+    timed(c, K_GRID, [&] {
+      if (c->plane) launch_plane_grid(c->d, c->stream);
+      else launch_grid(c->d, c->stream);
+    });
     for (int i = 0; i < nsteps; i++) {
       const StepMode m = i + 1 < nsteps ? STEP_AHEAD : STEP_LAST;
       if (graph) HIPCHECK(c, hipGraphLaunch(m == STEP_AHEAD ? c->graph_ahead : c->graph_last, c->stream));

@@ -271,6 +271,7 @@ void launch_plane_keys(const DevPtrs &d, void *stream);
 void launch_plane_p2g_only(const DevPtrs &d, void *stream);
 void launch_plane_g2p2g(const DevPtrs &d, void *stream);
 void launch_plane_grid(const DevPtrs &d, void *stream);
+void launch_plane_scan_final_grid(const DevPtrs &d, void *stream);  // as launch_scan_final_grid
 // plane contact step: plane_mb_grid, plane_mb_update, scan_partial (+ advance), scan_final,
 // place, mb_cells, plane_mb_claim, plane_p2g_only, plane_mb_p2g
 void launch_mb_cells(const DevPtrs &d, void *stream);  // objects per cell (both paths)

@@ -26,6 +26,7 @@
 #include "gy_device.hpp"
 #include "gy_internal.hpp"
 #include "gy_models.hpp"
+#include "gy_scan.hpp"
 
 namespace gy {
 namespace {
@@ -172,15 +173,25 @@ __device__ __forceinline__ void plane_brick(const DevPtrs &d, int gb, int lane, 
   if (lane == 0) d.brick_flag[gb] = 0;
 }
 
-__global__ __launch_bounds__(WG) void k_plane_grid(DevPtrs d) {
+// The plane grid update as block bid of nblk: one wave per listed brick.
+__device__ __forceinline__ void plane_grid_body(const DevPtrs &d, int bid, int nblk) {
   // the particle kernel that follows walks the active tile list from the start
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->next = 0;
+  if (bid == 0 && threadIdx.x == 0) d.ctrl->next = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = d.ctrl->active_bricks, step = d.ctrl->step, epoch = d.ctrl->epoch;
   int box[6];
   tile_box(d, epoch, box);  // plane tiles: x, y tile coordinates, one layer
-  for (int k = blockIdx.x * (WG / 64) + wave; k < n; k += gridDim.x * (WG / 64))
+  for (int k = bid * (WG / 64) + wave; k < n; k += nblk * (WG / 64))
     plane_brick(d, d.brick_list[k], lane, step, epoch, box);
+}
+
+__global__ __launch_bounds__(WG) void k_plane_grid(DevPtrs d) { plane_grid_body(d, blockIdx.x, gridDim.x); }
+
+// The second scan kernel of step s beside the plane grid update of step s + 1, as the 3D
+// k_scan_final_grid: blocks [0, ngrid) update the grid, the others scan.
+__global__ __launch_bounds__(WG) void k_plane_scan_final_grid(DevPtrs d, int ngrid) {
+  if ((int)blockIdx.x < ngrid) plane_grid_body(d, blockIdx.x, ngrid);
+  else scan_final_body(d, blockIdx.x - ngrid, gridDim.x - ngrid, false);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -532,6 +543,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int par = d.ctrl->parity;
   const int epoch_w = d.ctrl->epoch + (UPDATE ? 1 : 0);
   const TileQueue queue(d);
+  // the grid update before this kernel has read the brick list: the bricks this kernel
+  // flags are listed from an empty list (by either scan kernel)
+  if (blockIdx.x == 0 && tid == 0) d.ctrl->active_bricks = 0;
 
   for (;;) {
     if (tid == 0) s_tile = queue.tile(d, atomicAdd(&d.ctrl->next, 1));
@@ -905,9 +919,15 @@ void launch_plane_mb_g2p2g(const DevPtrs &d, void *st) {
     hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ELASTIC | KM_GK, true>), g, b, 0, S(st), d);
   else hipLaunchKernelGGL((k_plane_g2p2g<true, KM_ALL, true>), g, b, 0, S(st), d);
 }
+static int plane_grid_blocks(const DevPtrs &d) {
+  return std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
+}
 void launch_plane_grid(const DevPtrs &d, void *st) {
-  const int nb = std::max(1, std::min(blocks(d.nbricks, WG / 64), std::max(1, d.grid_factor) * d.persist_wgs));
-  hipLaunchKernelGGL(k_plane_grid, dim3(nb), dim3(WG), 0, S(st), d);
+  hipLaunchKernelGGL(k_plane_grid, dim3(plane_grid_blocks(d)), dim3(WG), 0, S(st), d);
+}
+void launch_plane_scan_final_grid(const DevPtrs &d, void *st) {
+  const int ng = plane_grid_blocks(d);
+  hipLaunchKernelGGL(k_plane_scan_final_grid, dim3(ng + d.nblocks_scan), dim3(WG), 0, S(st), d, ng);
 }
 
 }  // namespace gy

@@ -243,12 +243,13 @@ def test_graph_replay_matches_eager_launches():
     check(outs[1], outs[0], {"pos": 1e-7, "vel": 1e-5, "stress": 1e-4})
 
 
-def test_grid_update_beside_the_scan_matches_single_steps():
+@pytest.mark.parametrize("scene", ["dp_column_small", "dam_break_2d"])
+def test_grid_update_beside_the_scan_matches_single_steps(scene):
     """A call of several steps runs each later step's grid update in the same launch as the
-    step before's second scan kernel (graph replay and eager launches); a call of one step
-    runs it on its own. 60 one-step calls against one 60-step call, in both launch modes,
-    agree to rounding (the FP64 LDS sums are order-dependent)."""
-    sc = W.dp_column_small()
+    step before's second scan kernel (graph replay and eager launches; 3D and plane); a
+    call of one step runs it on its own. 60 one-step calls against one 60-step call, in both
+    launch modes, agree to rounding (the FP64 LDS sums are order-dependent)."""
+    sc = W.dp_column_small() if scene == "dp_column_small" else W.dam_break_2d(steps=60)
     for graph in (True, False):
         outs = []
         for single in (True, False):
