@@ -183,7 +183,10 @@ double gy_get_dt(const gy_ctx *ctx);
 int  gy_iterate(gy_ctx *ctx, double t, int32_t step);
 
 /* `nsteps` steps with t = t0 + i*dt, step = step0 + i (Solver::iterate_CPU_steps,
- * Solver.cpp:75-84, with the time passed explicitly). */
+ * Solver.cpp:75-84, with the time passed explicitly). Within one call of a single-body
+ * context without heat transfer, each later step's grid update runs in the same launch as
+ * the step before's binning scan; between calls the state is the same as after as many
+ * gy_iterate calls, and the results agree with them to rounding. */
 int  gy_iterate_n(gy_ctx *ctx, double t0, int32_t step0, int32_t nsteps);
 
 /* Wait for the context stream; reports a deferred device-side error (GY_ERR_RANGE). */
